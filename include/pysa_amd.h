@@ -1,0 +1,147 @@
+/*
+ * pysa_amd -- C ABI of the MI355X Metropolis / parallel-tempering engine.
+ *
+ * The reference (nasa/PySA) is pure Python + numba: its hot path is bound by
+ * passing numba functions by value, not through an FFI.  These entry points
+ * replace that path below the public Python API:
+ *
+ *   Solver.__init__ split/cast          pysa/sa.py:37-70, 204-205
+ *   simulation_parallel / _sequential   pysa/simulation.py:123-236
+ *     + random_sweep / sequential_sweep pysa/simulation.py:41-76
+ *     + ising/qubo update_spin          pysa/ising.py:40-63, pysa/qubo.py:40-63
+ *     + pt                              pysa/utils.py:26-58
+ *     + get_log_omega                   pysa/ais.py:79-112
+ *   per-read loop                       pysa/sa.py:425-427
+ *   ais_simulation_parallel/ais_sample  pysa/simulation.py:266-317
+ *   ising/qubo get_energy               pysa/ising.py:32-37, pysa/qubo.py:32-37
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative PYSA_AMD_E* code on
+ *     failure; pysa_amd_last_error() then holds a message (thread-local).
+ *   - Arrays are C-contiguous.  Host-pointer entry points copy in and out and
+ *     keep no pointer after they return.  *_device entry points take device
+ *     pointers and run on the given HIP stream (NULL = the problem's stream);
+ *     they return after the work has completed.
+ *   - dtype 0 = float32, 1 = float64 ("float_type" of the reference).
+ *     Energy and beta arrays use the problem's dtype; spin arrays are int8
+ *     holding +1/-1 (ising) or 1/0 (qubo).
+ *   - problem_type 0 = ising, 1 = qubo.
+ *   - order 0 = sequential, 1 = random (a fresh permutation per chain and
+ *     sweep), 2 = random_shared (one permutation per sweep shared by every
+ *     chain, the order the blocked form uses).
+ */
+#ifndef PYSA_AMD_H
+#define PYSA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYSA_AMD_OK 0
+#define PYSA_AMD_EINVAL (-1)  /* bad argument */
+#define PYSA_AMD_EHIP (-2)    /* HIP runtime error */
+#define PYSA_AMD_ENOTSUP (-3) /* valid request this build does not support */
+
+/* Engine forms accepted by pysa_amd_metropolis*. */
+#define PYSA_AMD_FORM_PLAIN 0
+#define PYSA_AMD_FORM_BLOCKED 1
+#define PYSA_AMD_FORM_AUTO 2
+
+/* Message describing the last failure on this thread ("" if none). */
+const char* pysa_amd_last_error(void);
+
+/* Library version string. */
+const char* pysa_amd_version(void);
+
+/*
+ * Upload one dense problem and keep it resident on the current device.
+ * J: n*n couplings with zero diagonal (row i holds J[i][:]); h: n local
+ * fields.  Replaces the split and cast of Solver.__init__ / metropolis_update
+ * (pysa/sa.py:67-70, 204-205): the caller passes the already-split arrays.
+ */
+int pysa_amd_problem_create(int32_t n, int32_t problem_type, int32_t dtype, const void* J,
+                            const void* h, int64_t* handle_out);
+
+/* Release a problem and its device memory. */
+int pysa_amd_problem_destroy(int64_t handle);
+
+/*
+ * Metropolis sweeps with optional parallel tempering for `reads` independent
+ * reads of R replicas (pysa/simulation.py:123-236 called once per read by
+ * pysa/sa.py:425-427).
+ *
+ *   S                sweeps per read (>= 1)
+ *   use_pt           replica exchange after every sweep (pysa/utils.py:26-58)
+ *   log_omega        compute the per-sweep log-omega (pysa/ais.py:79-112);
+ *                    requires a zero beta, validated by the caller
+ *   best_semantics   0 = per-replica best then argmin (simulation_parallel),
+ *                    1 = global best in replica order (simulation_sequential)
+ *   betas            R inverse temperatures in slot order (dtype)
+ *   beta_sort        R slot indices sorted by ascending beta (log_omega only)
+ *   seed             Philox key; sweep_offset shifts the sweep counter so a
+ *                    resumed run continues the same stream
+ *   form, block_b    engine form and blocked-form block size (0 = autotune)
+ *   replay_log_r     NULL, or reads*S*R*n log-uniforms indexed
+ *                    [read][sweep][replica][position] (replay mode)
+ *   replay_perm      NULL, or reads*S*R*n visit orders [read][sweep][replica][i]
+ *   replay_pt_logu   NULL, or reads*S*(R-1) PT log-uniforms [read][sweep][pair],
+ *                    pair 0 being (R-1, R-2)
+ *   states           in/out reads*R*n spins;  energies in/out reads*R (dtype)
+ *   beta_idx         out reads*R: slot -> replica map after the last sweep
+ *   best_state       out reads*n; best_energy out reads (dtype);
+ *   best_sweeps      out reads (0-based sweep of the best state)
+ *   log_omegas       out reads*S (f64) or NULL
+ *   accepted         out reads: accepted flips, or NULL
+ *   kernel_ms        out: device time of the run (HIP events), or NULL
+ */
+int pysa_amd_metropolis(int64_t handle, int32_t reads, int32_t R, int32_t S, int32_t order,
+                        int32_t use_pt, int32_t log_omega, int32_t best_semantics,
+                        const void* betas, const int32_t* beta_sort, uint64_t seed,
+                        uint64_t sweep_offset, int32_t form, int32_t block_b,
+                        const double* replay_log_r, const int32_t* replay_perm,
+                        const double* replay_pt_logu, int8_t* states, void* energies,
+                        int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                        int32_t* best_sweeps, double* log_omegas, uint64_t* accepted,
+                        double* kernel_ms);
+
+/* Same as pysa_amd_metropolis with every array (including betas, beta_sort
+ * and the replay arrays) already in device memory. */
+int pysa_amd_metropolis_device(int64_t handle, int32_t reads, int32_t R, int32_t S, int32_t order,
+                               int32_t use_pt, int32_t log_omega, int32_t best_semantics,
+                               const void* betas, const int32_t* beta_sort, uint64_t seed,
+                               uint64_t sweep_offset, int32_t form, int32_t block_b,
+                               const double* replay_log_r, const int32_t* replay_perm,
+                               const double* replay_pt_logu, int8_t* states, void* energies,
+                               int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                               int32_t* best_sweeps, double* log_omegas, uint64_t* accepted,
+                               double* kernel_ms, void* stream);
+
+/*
+ * Annealed importance sampling over `chains` independent chains
+ * (pysa/simulation.py:266-317 with update_spin_ref, pysa/ising.py:66-92).
+ * The caller passes J and h already scaled by the target beta
+ * (pysa/sa.py:511-512) in the problem handle.
+ *
+ *   temps        nt inverse temperatures (dtype), (t+1)/nt in the reference
+ *   reference    0 = uniform (pysa/ais.py:137-169), 1 = Bernoulli with
+ *                P(spin = -1 or 0) = p (pysa/ais.py:175-226)
+ *   ordering     0 = weight taken after the sweep at beta_t (reference,
+ *                pysa/simulation.py:305-316), 1 = before it (Neal 2001)
+ *   states       in/out chains*n spins
+ *   log_w        out chains log-weights (f64, without log Z0)
+ */
+int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, int32_t reference,
+                 double p, int32_t order, int32_t ordering, uint64_t seed,
+                 const double* replay_log_r, const int32_t* replay_perm, int8_t* states,
+                 double* log_w, double* kernel_ms);
+
+/* E = s.(J s / 2 + h) in f64 for `count` states (pysa/ising.py:32-37). */
+int pysa_amd_energy(int64_t handle, const int8_t* states, int64_t count, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYSA_AMD_H */

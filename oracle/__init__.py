@@ -1,0 +1,168 @@
+"""ORACLE -- test infrastructure only.
+
+CPU restatement of PySA's Metropolis / parallel-tempering path and AIS
+(oracle/oracle.cpp, built by oracle/Makefile into oracle/_build/liboracle.so)
+plus a pure-Python restatement for tiny replay cases (oracle/loops.py).  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use this
+package, and only as the checker / CPU baseline.
+
+What pins each oracle function (tests named in brackets):
+  philox           Random123 known-answer vectors      [test_oracle_rng]
+  det_log          |det_log(x) - log(x)| <= 2 ulp over (0,1) and wider,
+                   exact at powers of two               [test_oracle_rng]
+  perm (Feistel)   bijection of [0, n) for many n       [test_oracle_rng]
+  energy           brute force / closed forms (planted Wishart, all-ones)
+                                                        [test_oracle_semantics]
+  update_spin/sweep  beta=0 forced flips + E+dE consistency (the reference's
+                   own invariants, tests_ising_sequential.py:100-179),
+                   hand-worked 3-spin known answers, golden trajectories
+                   written by the reference under tests/golden/
+                                                        [test_oracle_semantics,
+                                                         test_oracle_golden]
+  pt               multiset invariance (utils tests), detailed balance of slot
+                   occupancy vs exact Boltzmann weights, golden trajectories
+  simulation best  brute-force lower bound (tests_*:227-290), golden vectors
+                   incl. the sequential aliasing quirk
+  get_log_omega    closed form on hand-made ladders, golden vectors
+  ais              exact log Z by enumeration (statistical), golden vectors
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc; no GPU needed)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        lib = ctypes.CDLL(LIB_PATH)
+        vp, i32, u64, i64, dbl = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64,
+                                  ctypes.c_int64, ctypes.c_double)
+        lib.oracle_philox.argtypes = [vp, vp, vp]
+        lib.oracle_philox.restype = None
+        lib.oracle_det_log.argtypes = [vp, vp, i64]
+        lib.oracle_det_log.restype = None
+        lib.oracle_perm.argtypes = [ctypes.c_uint32, vp, vp]
+        lib.oracle_perm.restype = None
+        lib.oracle_sweep_stream.argtypes = [u64, u64, u64, i32, i32, vp, vp]
+        lib.oracle_sweep_stream.restype = None
+        lib.oracle_energy_f64.argtypes = [i32, vp, vp, vp, i64, vp]
+        lib.oracle_energy_f64.restype = None
+        lib.oracle_metropolis.argtypes = ([i32] * 3 + [vp, vp] + [i32] * 7 + [vp, vp, u64, u64] +
+                                          [vp] * 3 + [vp, vp, vp, vp, vp, vp, vp, vp, i32])
+        lib.oracle_metropolis.restype = i32
+        lib.oracle_ais.argtypes = ([i32] * 3 + [vp, vp, vp] + [i32] * 3 + [dbl, i32, i32, u64] +
+                                   [vp, vp, vp, vp, i32])
+        lib.oracle_ais.restype = i32
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def philox(ctr, key):
+    out = np.zeros(4, dtype=np.uint32)
+    load().oracle_philox(_p(np.asarray(ctr, dtype=np.uint32)), _p(np.asarray(key, dtype=np.uint32)),
+                         _p(out))
+    return out
+
+
+def det_log(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    load().oracle_det_log(_p(x), _p(y), x.size)
+    return y
+
+
+def perm(n, keys):
+    out = np.empty(n, dtype=np.uint32)
+    load().oracle_perm(n, _p(np.asarray(keys, dtype=np.uint32)), _p(out))
+    return out
+
+
+def sweep_stream(seed, chain, sweep, n, order):
+    lr = np.empty(n, dtype=np.float64)
+    pm = np.empty(n, dtype=np.int32)
+    load().oracle_sweep_stream(seed, chain, sweep, n, order, _p(lr), _p(pm))
+    return lr, pm
+
+
+def energy_f64(J, h, states):
+    J = np.ascontiguousarray(J, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    s = np.ascontiguousarray(states, dtype=np.int8).reshape(-1, len(h))
+    out = np.empty(len(s), dtype=np.float64)
+    load().oracle_energy_f64(len(h), _p(J), _p(h), _p(s), len(s), _p(out))
+    return out
+
+
+def metropolis(J, h, ptype, betas, states, energies, sweeps, *, order=0, use_pt=True,
+               log_omega=False, best_sem=0, seed=0, sweep_offset=0, replay=None, nthreads=1):
+    """Run the oracle over all reads.  states: int8 (reads, R, n) (copied);
+    energies: (reads, R) in the float type of J.  Returns a dict of outputs."""
+    ft = np.dtype(J.dtype)
+    J = np.ascontiguousarray(J)
+    h = np.ascontiguousarray(h, dtype=ft)
+    betas = np.ascontiguousarray(betas, dtype=ft)
+    st = np.ascontiguousarray(states, dtype=np.int8).copy()
+    reads, R, n = st.shape
+    en = np.ascontiguousarray(energies, dtype=ft).copy()
+    bsort = np.argsort(betas, kind="quicksort").astype(np.int32)
+    bi = np.empty((reads, R), dtype=np.int64)
+    bs = np.empty((reads, n), dtype=np.int8)
+    be = np.empty(reads, dtype=ft)
+    bw = np.empty(reads, dtype=np.int32)
+    lw = np.zeros((reads, sweeps), dtype=np.float64)
+    acc = np.zeros(reads, dtype=np.uint64)
+    lr = pm = pt = None
+    if replay is not None:
+        lr = np.ascontiguousarray(replay["log_r"], dtype=np.float64)
+        if replay.get("perm") is not None:
+            pm = np.ascontiguousarray(replay["perm"], dtype=np.int32)
+        if replay.get("pt_logu") is not None:
+            pt = np.ascontiguousarray(replay["pt_logu"], dtype=np.float64)
+    rc = load().oracle_metropolis(n, ptype, 0 if ft == np.float32 else 1, _p(J), _p(h), reads, R,
+                                  sweeps, order, int(use_pt), int(log_omega), best_sem, _p(betas),
+                                  _p(bsort), seed & (2**64 - 1), sweep_offset, _p(lr), _p(pm),
+                                  _p(pt), _p(st), _p(en), _p(bi), _p(bs), _p(be), _p(bw), _p(lw),
+                                  _p(acc), nthreads)
+    assert rc == 0
+    return dict(states=st, energies=en, beta_idx=bi, best_state=bs, best_energy=be,
+                best_sweeps=bw, log_omegas=lw, accepted=acc)
+
+
+def ais(J, h, ptype, temps, states, *, ref=0, p=0.5, order=0, ordering=0, seed=0, replay=None,
+        nthreads=1):
+    ft = np.dtype(J.dtype)
+    J = np.ascontiguousarray(J)
+    h = np.ascontiguousarray(h, dtype=ft)
+    temps = np.ascontiguousarray(temps, dtype=ft)
+    st = np.ascontiguousarray(states, dtype=np.int8).copy()
+    chains, n = st.shape
+    lw = np.empty(chains, dtype=np.float64)
+    lr = pm = None
+    if replay is not None:
+        lr = np.ascontiguousarray(replay["log_r"], dtype=np.float64)
+        if replay.get("perm") is not None:
+            pm = np.ascontiguousarray(replay["perm"], dtype=np.int32)
+    rc = load().oracle_ais(n, ptype, 0 if ft == np.float32 else 1, _p(J), _p(h), _p(temps),
+                           len(temps), chains, ref, p, order, ordering, seed & (2**64 - 1),
+                           _p(lr), _p(pm), _p(st), _p(lw), nthreads)
+    assert rc == 0
+    return st, lw

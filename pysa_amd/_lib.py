@@ -1,0 +1,135 @@
+"""ctypes binding of libpysa_amd.so (the C ABI declared in include/pysa_amd.h).
+
+The library is built in-tree (``make -C pysa_amd/csrc``) and loaded from the
+package directory.  There is no CPU fallback: if the library or a GPU is
+missing, every compute entry point raises ``RuntimeError``.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpysa_amd.so")
+
+FORM_PLAIN, FORM_BLOCKED, FORM_AUTO = 0, 1, 2
+ORDER = {"sequential": 0, "random": 1, "random_shared": 2}
+
+_lib = None
+_lock = threading.Lock()
+
+_c_i32 = ctypes.c_int32
+_c_i64 = ctypes.c_int64
+_c_u64 = ctypes.c_uint64
+_vp = ctypes.c_void_p
+_c_dbl = ctypes.c_double
+
+# name -> argtypes; every function returns int32 status unless noted.
+_SIGNATURES = {
+    "pysa_amd_problem_create": [_c_i32, _c_i32, _c_i32, _vp, _vp, ctypes.POINTER(_c_i64)],
+    "pysa_amd_problem_destroy": [_c_i64],
+    "pysa_amd_metropolis": [_c_i64, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32,
+                            _vp, _vp, _c_u64, _c_u64, _c_i32, _c_i32, _vp, _vp, _vp,
+                            _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
+    "pysa_amd_metropolis_device": [_c_i64, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32,
+                                   _c_i32, _vp, _vp, _c_u64, _c_u64, _c_i32, _c_i32, _vp, _vp,
+                                   _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   ctypes.POINTER(_c_dbl), _vp],
+    "pysa_amd_ais": [_c_i64, _vp, _c_i32, _c_i32, _c_i32, _c_dbl, _c_i32, _c_i32, _c_u64,
+                     _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
+    "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
+}
+
+EXPORTED = sorted(list(_SIGNATURES) + ["pysa_amd_last_error", "pysa_amd_version"])
+
+
+def load(path=LIB_PATH):
+    """Load the shared library (once) and declare its signatures."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RuntimeError(
+                f"libpysa_amd.so not found at {path}; build it with `make -C pysa_amd/csrc` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(path)
+        for name, argtypes in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = _c_i32
+        lib.pysa_amd_last_error.restype = ctypes.c_char_p
+        lib.pysa_amd_last_error.argtypes = []
+        lib.pysa_amd_version.restype = ctypes.c_char_p
+        lib.pysa_amd_version.argtypes = []
+        _lib = lib
+        return lib
+
+
+def check(rc):
+    """Raise RuntimeError carrying the library's message for a failed call."""
+    if rc != 0:
+        msg = _lib.pysa_amd_last_error().decode() if _lib is not None else ""
+        raise RuntimeError(f"pysa_amd error {rc}: {msg}")
+
+
+def ptr(a):
+    """Data pointer of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the engine must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def dtype_code(float_type):
+    dt = np.dtype(float_type)
+    if dt == np.float32:
+        return 0
+    if dt == np.float64:
+        return 1
+    raise ValueError(f"float_type={float_type!r} not supported (float32 or float64).")
+
+
+class Problem:
+    """A dense problem resident on the current HIP device."""
+
+    def __init__(self, couplings, local_fields, problem_type, float_type):
+        lib = load()
+        self.float_type = np.dtype(float_type)
+        self.n = int(len(local_fields))
+        self.ptype = 0 if problem_type == "ising" else 1
+        self.J = np.ascontiguousarray(couplings, dtype=self.float_type)
+        self.h = np.ascontiguousarray(local_fields, dtype=self.float_type)
+        handle = _c_i64(0)
+        check(lib.pysa_amd_problem_create(self.n, self.ptype, dtype_code(self.float_type),
+                                          ptr(self.J), ptr(self.h), ctypes.byref(handle)))
+        self.handle = handle.value
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            load().pysa_amd_problem_destroy(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def energy(self, states):
+        """f64 energies of int8 states (count x n)."""
+        states = np.ascontiguousarray(states, dtype=np.int8).reshape(-1, self.n)
+        out = np.empty(len(states), dtype=np.float64)
+        check(load().pysa_amd_energy(self.handle, ptr(states), len(states), ptr(out)))
+        return out
+
+
+def blocked_available():
+    """Whether this build carries the MFMA-blocked form."""
+    return BLOCKED_READY
+
+
+# The blocked form is selected by form='auto' only once it is built.
+BLOCKED_READY = False

@@ -1,0 +1,67 @@
+// Shared declarations of the engine's kernels and launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pysa {
+
+enum Order : int { kSequential = 0, kRandom = 1, kRandomShared = 2 };
+
+// Device-side parameters of one Metropolis / parallel-tempering run over
+// `reads` independent reads of R replicas of one problem (plain form).
+template <typename T>
+struct PTParams {
+  int n, ptype, R, S, reads;
+  int order, use_pt, logw, best_sem;
+  uint32_t k0, k1;
+  uint64_t sweep_offset;
+  const T* Jt;       // n*n, row p holds the coefficients J[i][p] (column p of J)
+  const T* h;        // n
+  const T* betas;    // R, slot order (slot 0 = coldest)
+  const int* bsort;  // R, slots sorted by ascending beta (log-omega)
+  const double* rep_lr;   // reads*S*R*n or null
+  const int* rep_perm;    // reads*S*R*n or null
+  const double* rep_pt;   // reads*S*(R-1) or null
+  int8_t* spins;          // reads*R*n (+1/-1 ising, 1/0 qubo)
+  T* fields;              // reads*R*n, F = J s (without h)
+  T* energies;            // reads*R
+  int64_t* beta_idx;      // reads*R
+  int8_t* best_rep;       // reads*R*n (per-replica best states) or reads*n
+  T* best_rep_e;          // reads*R or reads
+  int* best_rep_sw;       // reads*R or reads
+  int8_t* best_state;     // reads*n
+  T* best_energy;         // reads
+  int* best_sweeps;       // reads
+  double* log_omegas;     // reads*S or null
+  unsigned long long* accepted;  // reads or null
+};
+
+// Annealed importance sampling over `chains` independent chains (plain form).
+template <typename T>
+struct AISParams {
+  int n, ptype, chains, nt, order, ref, ordering;
+  uint32_t k0, k1;
+  double lp, l1p;  // log p, log(1-p) of the Bernoulli reference
+  const T* Jt;
+  const T* h;
+  const T* temps;         // nt
+  const double* rep_lr;   // chains*nt*n or null
+  const int* rep_perm;    // chains*nt*n or null
+  int8_t* spins;          // chains*n
+  T* fields;              // chains*n
+  double* energy;         // chains, f64 running energy
+  double* logw;           // chains
+};
+
+template <typename T>
+hipError_t launch_energy_f64(const T* Jt, const T* h, const int8_t* spins, double* out, int n,
+                             long long count, hipStream_t st);
+template <typename T>
+hipError_t launch_plain_pt(const PTParams<T>& p, hipStream_t st);
+template <typename T>
+hipError_t launch_plain_ais(const AISParams<T>& p, hipStream_t st);
+
+// Largest problem size the plain kernels hold in registers.
+constexpr int kPlainMaxN = 4096;
+
+}  // namespace pysa
