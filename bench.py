@@ -1,0 +1,251 @@
+"""Benchmark: spin updates/s and TTS(99%) of the engine at N=1024.
+
+Workload (one "step"): the planted Wishart Ising instance, N=1024, alpha=0.5
+(M=512), float32 couplings, h=0; 512 reads x 32 PT replicas x 5000 sweeps,
+temperatures geometric 0.3 -> 1.5 (PySA's defaults), random update order.
+All inputs (couplings, initial states, ladders) are resident in HBM before
+the timed region; a step re-copies the initial states device-to-device and
+runs every read of the batch.
+
+Prints one JSON line.  `value` is the headline engine's whole-job spin
+updates/s (the MFMA-blocked form when this build carries it, else the plain
+form); both forms, their ratio, TTS(99%), the roofline of the dominant kernel
+and a CPU baseline (the oracle's C restatement of the reference algorithm on
+the host cores) ride along in the same line.
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+if _HERE not in sys.path:
+    sys.path.insert(0, _HERE)
+
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP8_PEAK_TFLOPS = 5000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def ladder(R, tmin=0.3, tmax=1.5, dtype=np.float32):
+    """pysa/sa.py:189-194 (geometric temperatures, slot 0 coldest)."""
+    ratio = np.exp(1 / (R - 1) * np.log(tmax / tmin))
+    return np.array([1 / (tmin * ratio**k) for k in range(R)], dtype=dtype)
+
+
+def tts(p, t_read, target=0.99):
+    if p <= 0:
+        return math.inf
+    if p >= target:
+        return t_read
+    return t_read * math.log(1 - target) / math.log(1 - p)
+
+
+class Workload:
+    """The benchmark batch, resident in HBM (library-owned device buffers)."""
+
+    def __init__(self, reads, R, S, seed=1003):
+        from pysa_amd import _lib, instances
+
+        self.lib = _lib
+        D = _lib.DeviceArray
+        self.n = 1024
+        self.reads, self.R, self.S = reads, R, S
+        P, self.e_gs = instances.planted_wishart(self.n, 0.5, seed)
+        self.J = P.copy()
+        np.fill_diagonal(self.J, 0)
+        self.h = np.diag(P).astype(np.float32).copy()
+        self.prob = _lib.Problem(self.J, self.h, "ising", np.float32)
+        rng = np.random.default_rng(seed)
+        st0 = (2 * rng.integers(0, 2, (reads, R, self.n)) - 1).astype(np.int8)
+        en0 = self.prob.energy(st0.reshape(-1, self.n)).astype(np.float32).reshape(reads, R)
+        self.betas = ladder(R)
+        bsort = np.argsort(self.betas, kind="quicksort").astype(np.int32)
+        self.d_st0, self.d_en0 = D.from_numpy(st0), D.from_numpy(en0)
+        self.d_betas, self.d_bsort = D.from_numpy(self.betas), D.from_numpy(bsort)
+        self.d_st = D(st0.shape, np.int8)
+        self.d_en = D(en0.shape, np.float32)
+        self.d_bi = D((reads, R), np.int64)
+        self.d_bs = D((reads, self.n), np.int8)
+        self.d_be = D(reads, np.float32)
+        self.d_bw = D(reads, np.int32)
+        self.d_acc = D(reads, np.uint64)
+        self.updates = float(self.n) * R * reads * S
+
+    def step(self, form, order, seed, block_b=0):
+        """One pass over the batch (synchronous); returns the device time (ms)
+        of the run, measured with HIP events on the launch stream."""
+        L = self.lib
+        self.d_st.copy_from(self.d_st0)
+        self.d_en.copy_from(self.d_en0)
+        kms = L._c_dbl(0.0)
+        L.check(L.load().pysa_amd_metropolis_device(
+            self.prob.handle, self.reads, self.R, self.S, order, 1, 0, 0,
+            self.d_betas.ptr, self.d_bsort.ptr, seed, 0, form, block_b, None, None, None,
+            self.d_st.ptr, self.d_en.ptr, self.d_bi.ptr, self.d_bs.ptr, self.d_be.ptr,
+            self.d_bw.ptr, None, self.d_acc.ptr, L.ctypes.byref(kms), None))
+        return kms.value
+
+    def success(self):
+        """Per-read f64 energy of best_state and relative gap to E_gs."""
+        bs = self.d_bs.numpy()
+        e = self.prob.energy(bs)
+        gap = (e - self.e_gs) / abs(self.e_gs)
+        return e, gap
+
+
+def run_form(wl, form_name, steps, warmup, seed):
+    L = wl.lib
+    form = {"plain": L.FORM_PLAIN, "blocked": L.FORM_BLOCKED}[form_name]
+    order = L.ORDER["random"] if form_name == "plain" else L.ORDER["random_shared"]
+    for w in range(warmup):
+        wl.step(form, order, seed + 7919 * (w + 1))
+        log(f"[bench] {form_name}: warmup step {w + 1}/{warmup} done")
+    # Each step call returns only after its device work has completed, so the
+    # wall clock below brackets exactly `steps` runs.
+    t0 = time.perf_counter()
+    kms, acc, gaps = [], 0, []
+    for k in range(steps):
+        kms.append(wl.step(form, order, seed + k))
+        acc += int(wl.d_acc.numpy().sum())
+        _, gap = wl.success()
+        gaps.append(gap)
+        log(f"[bench] {form_name}: step {k + 1}/{steps}: {kms[-1]:.1f} ms device")
+    wall = time.perf_counter() - t0
+    gaps = np.concatenate(gaps)
+    ms_step = 1e3 * wall / steps
+    t_read = np.mean(kms) * 1e-3 / wl.reads
+    p0 = float(np.mean(gaps <= 1e-6))
+    dev_s = float(np.sum(kms)) * 1e-3
+    out = {
+        # inputs resident in HBM: device time of the K runs (HIP events)
+        "value": wl.updates * steps / dev_s,
+        "ms_per_step": 1e3 * dev_s / steps,
+        "wall_ms_per_step": ms_step,
+        "kernel_ms_mean": float(np.mean(kms)),
+        "kernel_updates_per_s": wl.updates / (np.mean(kms) * 1e-3),
+        "acceptance": acc / (wl.updates * steps),
+        "p_success": p0,
+        "tts99_s": tts(p0, t_read),
+        "gap_tts99_s": {f"{g:g}": tts(float(np.mean(gaps <= g)), t_read) for g in (0.01, 0.05)},
+        "best_gap_median": float(np.median(gaps)),
+        "t_read_s": t_read,
+    }
+    return out
+
+
+def cpu_baseline(target_s=12.0, threads=None):
+    """The oracle's C restatement of the reference algorithm (O(N) coupling
+    row dot per proposal, replicas in parallel as numba.prange does) on the
+    host cores, over a bounded sample of the same workload."""
+    import oracle
+    from pysa_amd import instances
+
+    n, R = 1024, 32
+    P, _ = instances.planted_wishart(n, 0.5, 1003)
+    J = P.copy()
+    np.fill_diagonal(J, 0)
+    h = np.zeros(n, np.float32)
+    rng = np.random.default_rng(1003)
+    st = (2 * rng.integers(0, 2, (1, R, n)) - 1).astype(np.int8)
+    Jd = J.astype(np.float64)
+    en = np.array([[s @ Jd @ s / 2 for s in st[0].astype(np.float64)]], dtype=np.float32)
+    betas = ladder(R)
+    if threads is None:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    oracle.load()
+    oracle.metropolis(J, h, 0, betas, st, en, 5, order=1, seed=4, nthreads=threads)  # warm
+    t0 = time.perf_counter()
+    oracle.metropolis(J, h, 0, betas, st, en, 40, order=1, seed=5, nthreads=threads)
+    dt = time.perf_counter() - t0
+    sweeps = max(40, int(40 * target_s / max(dt, 1e-3)))
+    t0 = time.perf_counter()
+    oracle.metropolis(J, h, 0, betas, st, en, sweeps, order=1, seed=6, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n * R * sweeps / dt, "unit": "spin_updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 read x {R} replicas x {sweeps} sweeps of the same N=1024 Wishart "
+                      f"instance ({dt:.1f} s, random order, PT on)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=512)
+    ap.add_argument("--replicas", type=int, default=32)
+    ap.add_argument("--sweeps", type=int, default=5000)
+    ap.add_argument("--forms", default="auto", help="auto | plain | blocked | both")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1003)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine shards "
+              "reads across GPUs with no collective, so a multi-GPU figure is N x this one's.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from pysa_amd import _lib
+    forms = {"auto": ["plain", "blocked"] if _lib.blocked_available() else ["plain"],
+             "both": ["plain", "blocked"], "plain": ["plain"], "blocked": ["blocked"]}[args.forms]
+    wl = Workload(args.reads, args.replicas, args.sweeps, args.seed)
+    res = {}
+    for f in forms:
+        res[f] = run_form(wl, f, args.steps, args.warmup, args.seed)
+    head = "blocked" if "blocked" in res else "plain"
+    H = res[head]
+    n = wl.n
+    if head == "plain":
+        # one coupling row (n x 4 B) per accepted flip
+        bytes_per_ms = H["acceptance"] * wl.updates * n * 4 / H["kernel_ms_mean"]
+        achieved = bytes_per_ms / 1e6  # GB/s
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+        dtype = "f32"
+    else:
+        flop = wl.updates * 2 * n * 2  # split-bf16: two products per coupling
+        achieved = flop / (H["kernel_ms_mean"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None}
+        dtype = "bf16x2/f32"
+    line = {
+        "metric": "spin updates/s + TTS(99%) at N=1024: MFMA-blocked vs plain kernel, same GPU",
+        "value": H["value"],
+        "unit": "spin_updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": H["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": f"planted Wishart Ising N={n} alpha=0.5 float32, "
+                               f"{args.reads} reads x {args.replicas} replicas x {args.sweeps} "
+                               "sweeps, T 0.3->1.5, random order, PT",
+                   "form": head, "parallelism": "single GPU (reads across workgroups)"},
+        "roofline": roof,
+        "forms": res,
+    }
+    if "plain" in res and "blocked" in res:
+        line["blocked_over_plain"] = res["blocked"]["value"] / res["plain"]["value"]
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line, default=float))
+
+
+if __name__ == "__main__":
+    main()

@@ -137,6 +137,17 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
                  const double* replay_log_r, const int32_t* replay_perm, int8_t* states,
                  double* log_w, double* kernel_ms);
 
+/*
+ * Device-memory helpers for callers without their own HIP allocator (the
+ * benchmark keeps its inputs resident in HBM with these).  No counterpart in
+ * the reference, which has no device memory.
+ */
+int pysa_amd_malloc(uint64_t bytes, void** ptr_out);
+int pysa_amd_free(void* ptr);
+int pysa_amd_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int pysa_amd_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+int pysa_amd_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
+
 /* E = s.(J s / 2 + h) in f64 for `count` states (pysa/ising.py:32-37). */
 int pysa_amd_energy(int64_t handle, const int8_t* states, int64_t count, double* out);
 

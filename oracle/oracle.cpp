@@ -142,16 +142,20 @@ W4 ctr_perm(uint64_t chain, uint64_t sweep, uint32_t purpose) {
 }
 
 // ------------------------------------------------------------ problem -----
-// Coupling-row dot in the problem's float type, fixed left-to-right order in
-// eight interleaved partial sums (pysa/ising.py:48 `couplings[pos].dot(state)`).
+// Coupling-row dot in the problem's float type, fixed order: 32 interleaved
+// partial sums (four independent 8-wide vector accumulators) combined
+// pairwise (pysa/ising.py:48 `couplings[pos].dot(state)`).
 template <typename T>
 T row_dot(const T* row, const T* s, int n) {
-  T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  T acc[32];
+  for (int q = 0; q < 32; ++q) acc[q] = 0;
   int j = 0;
-  for (; j + 8 <= n; j += 8)
-    for (int q = 0; q < 8; ++q) acc[q] += row[j + q] * s[j + q];
+  for (; j + 32 <= n; j += 32)
+    for (int q = 0; q < 32; ++q) acc[q] += row[j + q] * s[j + q];
   for (int q = 0; j < n; ++j, ++q) acc[q] += row[j] * s[j];
-  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  for (int w = 16; w >= 1; w >>= 1)
+    for (int q = 0; q < w; ++q) acc[q] = acc[q] + acc[q + w];
+  return acc[0];
 }
 
 struct RunCfg {
@@ -249,7 +253,7 @@ int metropolis(const RunCfg& c, const T* J, const T* h, const T* betas, const in
     std::vector<T> gst;
     for (int s = 0; s < S; ++s) {
       for (int k = 0; k < R; ++k) bs[bi[k]] = betas[k];  // simulation.py:151-152
-#pragma omp parallel for schedule(dynamic, 1) if (c.nthreads != 1)
+#pragma omp parallel for schedule(static) if (c.nthreads != 1)
       for (int k = 0; k < R; ++k) {  // simulation.py:154 prange
         const double d = sweep_one<T>(c, J, h, &st[(size_t)k * n], bs[k], read, k, s, &acc[k]);
         E[k] = (T)((double)E[k] + d);  // simulation.py:157-158

@@ -40,6 +40,11 @@ _SIGNATURES = {
     "pysa_amd_ais": [_c_i64, _vp, _c_i32, _c_i32, _c_i32, _c_dbl, _c_i32, _c_i32, _c_u64,
                      _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
+    "pysa_amd_malloc": [_c_u64, ctypes.POINTER(_vp)],
+    "pysa_amd_free": [_vp],
+    "pysa_amd_memcpy_h2d": [_vp, _vp, _c_u64],
+    "pysa_amd_memcpy_d2h": [_vp, _vp, _c_u64],
+    "pysa_amd_memcpy_d2d": [_vp, _vp, _c_u64],
 }
 
 EXPORTED = sorted(list(_SIGNATURES) + ["pysa_amd_last_error", "pysa_amd_version"])
@@ -124,6 +129,51 @@ class Problem:
         out = np.empty(len(states), dtype=np.float64)
         check(load().pysa_amd_energy(self.handle, ptr(states), len(states), ptr(out)))
         return out
+
+
+class DeviceArray:
+    """A device buffer owned by the library's HIP runtime, shaped like a
+    numpy array (for callers that keep inputs resident in HBM)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if np.ndim(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        check(load().pysa_amd_malloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(load().pysa_amd_memcpy_h2d(self.ptr, ptr(a), self.nbytes))
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        check(load().pysa_amd_memcpy_d2d(self.ptr, other.ptr, self.nbytes))
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(load().pysa_amd_memcpy_d2h(ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            load().pysa_amd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def blocked_available():

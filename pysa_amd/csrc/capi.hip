@@ -154,6 +154,7 @@ int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
   p.use_pt = a.use_pt;
   p.logw = a.logw;
   p.best_sem = a.best_sem;
+  p.refresh = 64;
   p.k0 = (uint32_t)a.seed;
   p.k1 = (uint32_t)(a.seed >> 32);
   p.sweep_offset = a.sweep_offset;
@@ -469,6 +470,37 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   if (kernel_ms) *kernel_ms = t;
   HIP_TRY(hipMemcpy(states, dS, (size_t)chains * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(log_w, dW, (size_t)chains * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pysa_amd_malloc(uint64_t bytes, void** ptr_out) {
+  g_err.clear();
+  if (!ptr_out) return fail(PYSA_AMD_EINVAL, "null argument");
+  HIP_TRY(hipMalloc(ptr_out, bytes ? bytes : 1));
+  return 0;
+}
+
+int pysa_amd_free(void* ptr) {
+  g_err.clear();
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return 0;
+}
+
+int pysa_amd_memcpy_h2d(void* dst, const void* src, uint64_t bytes) {
+  g_err.clear();
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pysa_amd_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
+  g_err.clear();
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pysa_amd_memcpy_d2d(void* dst, const void* src, uint64_t bytes) {
+  g_err.clear();
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   return 0;
 }
 

@@ -13,6 +13,7 @@ template <typename T>
 struct PTParams {
   int n, ptype, R, S, reads;
   int order, use_pt, logw, best_sem;
+  int refresh;  // recompute local fields in f64 every `refresh` sweeps (0 = only at start)
   uint32_t k0, k1;
   uint64_t sweep_offset;
   const T* Jt;       // n*n, row p holds the coefficients J[i][p] (column p of J)

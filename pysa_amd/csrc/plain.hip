@@ -8,14 +8,17 @@
 //   pysa/ais.py:79-112          get_log_omega
 //   pysa/simulation.py:266-317  ais_simulation_parallel / ais_sample
 //
-// Layout: one workgroup per read; each wave owns one replica at a time.  The
-// replica's local fields F = J s live in VGPRs (lane l holds F[64m + l] in
-// register m) and its spins as one bit per register slot.  A proposal reads
-// F[pos] with a readlane, so dE is O(1); an accepted flip streams one
-// coupling row (coalesced, 64 lanes x 4 bytes per step) and updates every
-// field.  The sweep loop, best tracking, parallel tempering and log-omega
-// run inside one persistent launch; replicas of a read meet at a workgroup
-// barrier once per sweep.
+// Layout.  One workgroup per read (PT) or per group of chains (AIS); each
+// wave carries one chain at a time.  A chain's local fields F = J s live in VGPRs:
+// element i sits in lane (i>>2)&63, register slot ((i>>8)<<2)|(i&3), so four
+// consecutive slots of a lane hold four consecutive elements and a coupling
+// row moves as 16-byte vectors.  Spins are one bit per slot.  A proposal reads
+// F[pos] with a readlane, so dE is O(1); an accepted flip adds one coupling row
+// (times dS in {+-1, +-2}) to every field.
+//
+// The sweep loop, best tracking, parallel tempering and log-omega run inside
+// one persistent launch; the replicas of a read meet at a workgroup barrier
+// once per sweep.
 #include "engine.hpp"
 #include "rng.hpp"
 
@@ -25,6 +28,9 @@ namespace {
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 __device__ __forceinline__ float rl(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ uint32_t rl(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 __device__ __forceinline__ uint64_t rl(uint64_t v, int lane) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
@@ -46,65 +52,103 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
-// One chain held by one wave.
-template <typename T, int CH>
+// Element <-> (lane, slot) map of the field layout.
+__device__ __forceinline__ int elem_of(int lane, int g) {
+  return ((g >> 2) << 8) + (lane << 2) + (g & 3);
+}
+__device__ __forceinline__ int lane_of(int i) { return (i >> 2) & 63; }
+__device__ __forceinline__ int slot_of(int i) { return ((i >> 8) << 2) | (i & 3); }
+
+template <typename T>
+struct alignas(4 * sizeof(T)) T4 {
+  T v[4];
+};
+
+// One chain: G field registers per lane (G a multiple of 4, n <= 64*G).
+template <int G>
+struct BitsFor {
+  using type = uint64_t;
+};
+template <>
+struct BitsFor<4> {
+  using type = uint32_t;
+};
+template <>
+struct BitsFor<8> {
+  using type = uint32_t;
+};
+template <>
+struct BitsFor<16> {
+  using type = uint32_t;
+};
+template <>
+struct BitsFor<32> {
+  using type = uint32_t;
+};
+
+template <typename T, int G>
 struct Chain {
-  T f[CH];        // f[m] at lane l = F[64m + l]
-  uint64_t bits;  // bit m at lane l: spin 64m + l is up (+1 ising / 1 qubo)
+  using B = typename BitsFor<G>::type;
+  T f[G];
+  B bits;  // bit g: the spin of element elem_of(lane, g) is up (+1 / 1)
 
   __device__ void load(const T* F, const int8_t* s, int n) {
     const int lane = threadIdx.x & 63;
     bits = 0;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const int i = 64 * m + lane;
-      f[m] = (i < n) ? F[i] : T(0);
-      if (i < n && s[i] == 1) bits |= (1ull << m);
+    for (int g = 0; g < G; ++g) {
+      const int i = elem_of(lane, g);
+      f[g] = (i < n) ? F[i] : T(0);
+      if (i < n && s[i] == 1) bits |= (B(1) << g);
     }
   }
   __device__ void store(T* F, int8_t* s, int n, int ptype) const {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const int i = 64 * m + lane;
+    for (int g = 0; g < G; ++g) {
+      const int i = elem_of(lane, g);
       if (i < n) {
-        F[i] = f[m];
-        s[i] = ((bits >> m) & 1) ? int8_t(1) : (ptype == 0 ? int8_t(-1) : int8_t(0));
+        F[i] = f[g];
+        s[i] = ((bits >> g) & 1) ? int8_t(1) : (ptype == 0 ? int8_t(-1) : int8_t(0));
       }
     }
   }
   __device__ void store_spins(int8_t* s, int n, int ptype) const {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const int i = 64 * m + lane;
-      if (i < n) s[i] = ((bits >> m) & 1) ? int8_t(1) : (ptype == 0 ? int8_t(-1) : int8_t(0));
+    for (int g = 0; g < G; ++g) {
+      const int i = elem_of(lane, g);
+      if (i < n) s[i] = ((bits >> g) & 1) ? int8_t(1) : (ptype == 0 ? int8_t(-1) : int8_t(0));
     }
   }
+  __device__ bool up_at(int pos) const { return (rl(bits, lane_of(pos)) >> slot_of(pos)) & 1; }
 
-  // F = J s accumulated in f64 and rounded once; returns E = s.(F/2 + h) in
-  // f64 (the periodic recomputation that bounds drift).
-  __device__ double refresh(const T* __restrict__ Jt, const T* __restrict__ h, int n, int ptype) {
+  // F = J s accumulated in f64 and rounded once to T; returns
+  // E = s.(F/2 + h) in f64 (the periodic recomputation that bounds drift).
+  __device__ double refresh(const T* __restrict__ Jt, const T* h, int n, int ptype) {
     const int lane = threadIdx.x & 63;
     double e = 0.0;
 #pragma unroll
-    for (int m = 0; m < CH; ++m) {
-      const int i = 64 * m + lane;
-      double a = 0.0;
-      for (int jc = 0; 64 * jc < n; ++jc) {
-        const int jl = 64 * jc + lane;
-        const int sv_l = (jl < n) ? (((bits >> jc) & 1) ? 1 : (ptype == 0 ? -1 : 0)) : 0;
-        const int cnt = min(64, n - 64 * jc);
-        for (int t = 0; t < cnt; ++t) {
-          const int sj = rl(sv_l, t);
-          if (sj == 0) continue;
-          if (i < n) a += (double)Jt[(size_t)(64 * jc + t) * n + i] * (double)sj;
-        }
+    for (int g4 = 0; g4 < G / 4; ++g4) {
+      double a[4] = {0.0, 0.0, 0.0, 0.0};
+      const int i0 = (g4 << 8) + (lane << 2);
+      for (int j = 0; j < n; ++j) {
+        const bool upj = up_at(j);
+        if (ptype == 1 && !upj) continue;
+        const double sj = upj ? 1.0 : -1.0;
+        const T* row = Jt + (size_t)j * n;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (i0 + q < n) a[q] += (double)row[i0 + q] * sj;
       }
-      f[m] = (T)a;
-      if (i < n) {
-        const double si = ((bits >> m) & 1) ? 1.0 : (ptype == 0 ? -1.0 : 0.0);
-        e += si * (0.5 * a + (double)h[i]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int g = 4 * g4 + q;
+        f[g] = (T)a[q];
+        if (i0 + q < n) {
+          const double si = ((bits >> g) & 1) ? 1.0 : (ptype == 0 ? -1.0 : 0.0);
+          e += si * (0.5 * a[q] + (double)h[i0 + q]);
+        }
       }
     }
     return wave_sum(e);
@@ -114,96 +158,149 @@ struct Chain {
     const int lane = threadIdx.x & 63;
     int c = 0;
 #pragma unroll
-    for (int m = 0; m < CH; ++m)
-      if (64 * m + lane < n) c += (int)((bits >> m) & 1);
+    for (int g = 0; g < G; ++g)
+      if (elem_of(lane, g) < n) c += (int)((bits >> g) & 1);
     return wave_sum(c);
   }
 
-  // One Metropolis sweep at inverse temperature beta
-  // (simulation.py:60-76 / 41-57 with ising.py:40-63 / qubo.py:40-63).
-  // AIS adds the reference-distribution term of update_spin_ref
-  // (ising.py:74-83).  Returns the f64 sum of accepted energy changes.
-  template <bool AIS>
-  __device__ double sweep(int n, int ptype, const T* __restrict__ Jt, const T* __restrict__ h,
-                          T beta, uint64_t chain, uint64_t sweep_ctr, int order, uint32_t k0,
-                          uint32_t k1, const double* __restrict__ rep_lr,
-                          const int* __restrict__ rep_perm, int ref, double lratio,
-                          unsigned long long& acc, int& ones_delta) {
+  // Add dS times a staged row (LDS, 16-byte aligned, zero-padded to 64*G).
+  __device__ __forceinline__ void add_row(const T* row, T dS) {
     const int lane = threadIdx.x & 63;
-    FeistelKey fk;
-    if (order != kSequential && rep_perm == nullptr)
-      fk = feistel_make((uint32_t)n, perm_keys(order == kRandomShared ? 0ull : chain, sweep_ctr,
-                                               order == kRandomShared ? kStreamShared : kStreamPerm,
-                                               k0, k1));
-    const double bd = (double)beta;
-    double sum = 0.0;
-    for (int c = 0; c < CH; ++c) {
-      if (64 * c >= n) break;
-      const int i = 64 * c + lane;
-      int pos_l = 0;
-      double lr_l = 0.0;
-      if (i < n) {
-        if (rep_perm)
-          pos_l = rep_perm[i];
-        else
-          pos_l = (order == kSequential) ? i : (int)feistel_perm(fk, (uint32_t)i);
-        if (rep_lr)
-          lr_l = rep_lr[pos_l];
-        else
-          lr_l = log_u32(pick_word(sweep_block(chain, sweep_ctr, (uint32_t)pos_l, k0, k1),
-                                   (uint32_t)pos_l & 3u));
-      }
-      const int cnt = min(64, n - 64 * c);
-      for (int t = 0; t < cnt; ++t) {
-        const int pos = rl(pos_l, t);
-        const double lr = rl(lr_l, t);
-        const int m = pos >> 6, ln = pos & 63;
-        const T fv = rl(f[m], ln);
-        const bool up = (rl(bits, ln) >> m) & 1;
-        const T fh = fv + h[pos];
-        const double dn = (ptype == 0) ? 2.0 * (up ? 1.0 : -1.0) * (double)fh
-                                       : (2.0 * (up ? 1.0 : 0.0) - 1.0) * (double)fh;
-        bool accept;
-        if (AIS) {
-          const double refd = (ref == 1) ? (up ? lratio : -lratio) : 0.0;
-          const double full = bd * dn + (1.0 - bd) * refd;
-          accept = full >= 0 || lr < full;
-        } else {
-          accept = dn >= 0 || lr < bd * dn;
-        }
-        if (accept) {
-          sum += -dn;
-          ++acc;
-          ones_delta += up ? -1 : 1;
-          const T dS = (ptype == 0) ? (up ? T(-2) : T(2)) : (up ? T(-1) : T(1));
-          if (lane == ln) bits ^= (1ull << m);
-          const T* row = Jt + (size_t)pos * n;
+    const T4<T>* r4 = reinterpret_cast<const T4<T>*>(row);
 #pragma unroll
-          for (int mm = 0; mm < CH; ++mm) {
-            const int idx = 64 * mm + lane;
-            if (idx < n) f[mm] += row[idx] * dS;
-          }
-        }
-      }
+    for (int g4 = 0; g4 < G / 4; ++g4) {
+      const T4<T> x = r4[(g4 << 6) + lane];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) f[4 * g4 + q] += x.v[q] * dS;
     }
-    return sum;
+  }
+  // Add dS times a global row of length n.
+  __device__ __forceinline__ void add_row_global(const T* __restrict__ row, int n, bool vec, T dS) {
+    const int lane = threadIdx.x & 63;
+    if (vec) {  // n % 4 == 0: the row start is 16-byte aligned
+      const T4<T>* r4 = reinterpret_cast<const T4<T>*>(row);
+      T4<T> x[G / 4];
+#pragma unroll
+      for (int g4 = 0; g4 < G / 4; ++g4) {
+        const int i0 = (g4 << 8) + (lane << 2);
+        if (i0 < n)
+          x[g4] = r4[(g4 << 6) + lane];
+        else
+          x[g4] = T4<T>{{T(0), T(0), T(0), T(0)}};
+      }
+#pragma unroll
+      for (int g4 = 0; g4 < G / 4; ++g4)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[4 * g4 + q] += x[g4].v[q] * dS;
+    } else {
+      T x[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int i = elem_of(lane, g);
+        x[g] = (i < n) ? row[i] : T(0);
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) f[g] += x[g] * dS;
+    }
   }
 };
 
-constexpr int kMaxR = 1024;
-
-// Waves per workgroup for a chain of CH field registers of type T: keeps the
-// fields plus working registers inside the VGPR budget at that occupancy.
-template <typename T, int CH>
-constexpr int waves_for() {
-  return (CH * (int)(sizeof(T) / 4) <= 16) ? 16 : (CH * (int)(sizeof(T) / 4) <= 32) ? 8 : 4;
+// Decision for one proposal (ising.py:48-52 / qubo.py:48-52; with AIS the
+// reference-distribution term of update_spin_ref, ising.py:75-83).
+template <typename T, bool AIS>
+__device__ __forceinline__ bool decide(int ptype, bool up, T fh, double lr, double bd, int ref,
+                                       double lratio, double& dn) {
+  dn = (ptype == 0) ? 2.0 * (up ? 1.0 : -1.0) * (double)fh
+                    : (2.0 * (up ? 1.0 : 0.0) - 1.0) * (double)fh;
+  if (AIS) {
+    const double refd = (ref == 1) ? (up ? lratio : -lratio) : 0.0;
+    const double full = bd * dn + (1.0 - bd) * refd;
+    return full >= 0 || lr < full;
+  }
+  return dn >= 0 || lr < bd * dn;
 }
 
-template <typename T, int CH>
-__global__ __launch_bounds__((64 * waves_for<T, CH>())) void plain_pt_kernel(PTParams<T> p) {
+// Log-uniform of (chain, position) from replay or Philox.
+__device__ __forceinline__ double log_uniform(const double* rep, uint64_t chain, uint64_t sw,
+                                              int pos, uint32_t k0, uint32_t k1) {
+  if (rep) return rep[pos];
+  return log_u32(pick_word(sweep_block(chain, sw, (uint32_t)pos, k0, k1), (uint32_t)pos & 3u));
+}
+
+constexpr int kMaxR = 1024;
+
+// Waves per workgroup for G field registers of type T, keeping fields and
+// working registers inside the VGPR budget.
+template <typename T, int G>
+constexpr int kernel_waves() {
+  return G * (int)(sizeof(T) / 4) <= 16 ? 16 : G * (int)(sizeof(T) / 4) <= 32 ? 8 : 4;
+}
+
+
+struct SweepCtx {
+  int n, ptype, order, ref;
+  uint32_t k0, k1;
+  uint64_t sw;
+  double lratio;
+};
+
+// One sweep of one chain (simulation.py:41-76): visit order from the
+// sequential order, a Feistel permutation (per chain, or shared by all chains
+// of the sweep) or a replayed order; an accepted flip loads its coupling row
+// from global memory (L2 / Infinity Cache resident) as 16-byte vectors.
+template <typename T, int G, bool AIS>
+__device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const SweepCtx& c,
+                               const T* __restrict__ Jt, T beta, uint64_t cid,
+                               const double* rep_lr, const int* rep_perm,
+                               unsigned long long& acc, int& od) {
+  const int n = c.n, lane = threadIdx.x & 63;
+  const bool vec = (n & 3) == 0;
+  od = 0;
+  if (!active) return 0.0;
+  FeistelKey fk;
+  if (c.order != kSequential && rep_perm == nullptr)
+    fk = feistel_make((uint32_t)n,
+                      perm_keys(c.order == kRandomShared ? 0ull : cid, c.sw,
+                                c.order == kRandomShared ? kStreamShared : kStreamPerm, c.k0, c.k1));
+  double sum = 0.0;
+  for (int w = 0; w < n; w += 64) {
+    const int vi = w + lane;
+    int pos_l = 0;
+    double lr_l = 0.0;
+    if (vi < n) {
+      if (rep_perm)
+        pos_l = rep_perm[vi];
+      else
+        pos_l = (c.order == kSequential) ? vi : (int)feistel_perm(fk, (uint32_t)vi);
+      lr_l = log_uniform(rep_lr, cid, c.sw, pos_l, c.k0, c.k1);
+    }
+    const int cnt = min(64, n - w);
+    for (int t = 0; t < cnt; ++t) {
+      const int pos = rl(pos_l, t);
+      const double lr = rl(lr_l, t);
+      const int pl = lane_of(pos), ps = slot_of(pos);
+      const bool up = (rl(ch.bits, pl) >> ps) & 1;
+      const T fh = rl(ch.f[ps], pl) + hs[pos];
+      double dn;
+      if (decide<T, AIS>(c.ptype, up, fh, lr, (double)beta, c.ref, c.lratio, dn)) {
+        sum += -dn;
+        ++acc;
+        od += up ? -1 : 1;
+        if (lane == pl) ch.bits ^= (typename Chain<T, G>::B(1) << ps);
+        ch.add_row_global(Jt + (size_t)pos * n, n, vec,
+                          (c.ptype == 0) ? (up ? T(-2) : T(2)) : (up ? T(-1) : T(1)));
+      }
+    }
+  }
+  return sum;
+}
+
+template <typename T, int G>
+__global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(PTParams<T> p) {
   const int read = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
   const int R = p.R, n = p.n;
+  __shared__ T s_h[kPlainMaxN];
   __shared__ T s_bs[kMaxR];
   __shared__ T s_beta[kMaxR];
   __shared__ T s_E[kMaxR];
@@ -213,6 +310,7 @@ __global__ __launch_bounds__((64 * waves_for<T, CH>())) void plain_pt_kernel(PTP
   __shared__ T s_gE;
   __shared__ int s_gsw, s_improved;
 
+  for (int i = tid; i < n; i += blockDim.x) s_h[i] = p.h[i];
   for (int k = tid; k < R; k += blockDim.x) {
     s_bi[k] = p.beta_idx[(size_t)read * R + k];
     s_E[k] = p.energies[(size_t)read * R + k];
@@ -223,19 +321,29 @@ __global__ __launch_bounds__((64 * waves_for<T, CH>())) void plain_pt_kernel(PTP
     s_gsw = 0;
     s_improved = 0;
   }
-  // Local fields of every replica, F = J s in f64 rounded once.
-  for (int r = wave; r < R; r += W) {
-    Chain<T, CH> ch;
-    const size_t off = ((size_t)read * R + r) * n;
-    ch.load(p.fields + off, p.spins + off, n);
-    ch.refresh(p.Jt, p.h, n, p.ptype);
-    ch.store(p.fields + off, p.spins + off, n, p.ptype);
-  }
   __syncthreads();
-
+  SweepCtx c;
+  c.n = n;
+  c.ptype = p.ptype;
+  c.order = p.order;
+  c.ref = 0;
+  c.k0 = p.k0;
+  c.k1 = p.k1;
+  c.lratio = 0.0;
   unsigned long long acc = 0;
   for (int s = 0; s < p.S; ++s) {
-    const uint64_t sw = (uint64_t)s + p.sweep_offset;
+    c.sw = (uint64_t)s + p.sweep_offset;
+    // Local fields F = J s recomputed in f64 (rounded once) at the start and
+    // every `refresh` sweeps, which bounds the drift of float couplings.
+    if (s == 0 || (p.refresh > 0 && s % p.refresh == 0)) {
+      for (int r = wave; r < R; r += W) {
+        Chain<T, G> ch;
+        const size_t off = ((size_t)read * R + r) * n;
+        ch.load(p.fields + off, p.spins + off, n);
+        ch.refresh(p.Jt, s_h, n, p.ptype);
+        ch.store(p.fields + off, p.spins + off, n, p.ptype);
+      }
+    }
     for (int k = tid; k < R; k += blockDim.x) s_bs[s_bi[k]] = s_beta[k];  // simulation.py:151-152
     // PT log-uniforms, one per pair, drawn in parallel ahead of the serial walk.
     if (p.use_pt)
@@ -244,20 +352,21 @@ __global__ __launch_bounds__((64 * waves_for<T, CH>())) void plain_pt_kernel(PTP
         if (p.rep_pt)
           lu = p.rep_pt[((size_t)read * p.S + s) * (R - 1) + k];
         else
-          lu = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)k, p.k0, p.k1), (uint32_t)k & 3u));
+          lu = log_u32(pick_word(pt_block((uint64_t)read, c.sw, (uint32_t)k, p.k0, p.k1),
+                                 (uint32_t)k & 3u));
         s_ptlu[k] = lu;
       }
     __syncthreads();
-    for (int r = wave; r < R; r += W) {  // simulation.py:154-164
-      Chain<T, CH> ch;
+    for (int r = wave; r < R; r += W) {  // replicas of this read (simulation.py:154-164)
+      Chain<T, G> ch;
       const size_t off = ((size_t)read * R + r) * n;
       ch.load(p.fields + off, p.spins + off, n);
       const size_t rb = (((size_t)read * p.S + s) * R + r) * (size_t)n;
-      int od = 0;
-      const double d = ch.template sweep<false>(
-          n, p.ptype, p.Jt, p.h, s_bs[r], (uint64_t)read * R + r, sw, p.order, p.k0, p.k1,
-          p.rep_lr ? p.rep_lr + rb : nullptr, p.rep_perm ? p.rep_perm + rb : nullptr, 0, 0.0, acc,
-          od);
+      int od;
+      const double d = sweep_direct<T, G, false>(ch, true, s_h, c, p.Jt, s_bs[r],
+                                                 (uint64_t)read * R + r,
+                                                 p.rep_lr ? p.rep_lr + rb : nullptr,
+                                                 p.rep_perm ? p.rep_perm + rb : nullptr, acc, od);
       ch.store(p.fields + off, p.spins + off, n, p.ptype);
       const T en = (T)((double)s_E[r] + d);  // simulation.py:157-158
       if (lane == 0) s_E[r] = en;
@@ -351,35 +460,45 @@ __global__ __launch_bounds__((64 * waves_for<T, CH>())) void plain_pt_kernel(PTP
   }
 }
 
-// AIS: one wave per chain, no barriers (simulation.py:266-317).
-template <typename T, int CH>
-__global__ __launch_bounds__(256) void plain_ais_kernel(AISParams<T> p) {
-  const int lane = threadIdx.x & 63;
-  const int chain = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (chain >= p.chains) return;
+// AIS: each wave carries one chain through the whole beta schedule
+// (simulation.py:266-317); chains are independent, so there are no barriers
+// after h is staged.
+template <typename T, int G>
+__global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_ais_kernel(AISParams<T> p) {
+  __shared__ T s_h[kPlainMaxN];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
   const int n = p.n;
-  Chain<T, CH> ch;
-  const size_t off = (size_t)chain * n;
-  ch.load(p.fields + off, p.spins + off, n);
-  double E = ch.refresh(p.Jt, p.h, n, p.ptype);
+  for (int i = tid; i < n; i += blockDim.x) s_h[i] = p.h[i];
+  __syncthreads();
+  const int chain = blockIdx.x * W + wave;
+  if (chain >= p.chains) return;
+  Chain<T, G> ch;
+  ch.load(p.fields + (size_t)chain * n, p.spins + (size_t)chain * n, n);
+  double E = ch.refresh(p.Jt, s_h, n, p.ptype);
   int ones = ch.count_up(n);
-  const double lratio = p.lp - p.l1p;
-  double lw = 0.0, bold = 0.0;
+  SweepCtx c;
+  c.n = n;
+  c.ptype = p.ptype;
+  c.order = p.order;
+  c.ref = p.ref;
+  c.k0 = p.k0;
+  c.k1 = p.k1;
+  c.lratio = p.lp - p.l1p;
+  double bold = 0.0, lw = 0.0;
   unsigned long long acc = 0;
   for (int t = 0; t < p.nt; ++t) {
+    c.sw = (uint64_t)t;
     const T beta = p.temps[t];
     const double bd = (double)beta;
-    if (p.ordering == 1) {
+    if (p.ordering == 1) {  // Neal's ordering: weight before the transition
       const double refE = (p.ref == 1) ? -((double)(n - ones) * p.lp + (double)ones * p.l1p) : 0.0;
       lw += (-bd * E - (1.0 - bd) * refE) - (-bold * E - (1.0 - bold) * refE);
     }
     const size_t rb = ((size_t)chain * p.nt + t) * (size_t)n;
-    int od = 0;
-    const double d = ch.template sweep<true>(n, p.ptype, p.Jt, p.h, beta, (uint64_t)chain,
-                                             (uint64_t)t, p.order, p.k0, p.k1,
-                                             p.rep_lr ? p.rep_lr + rb : nullptr,
-                                             p.rep_perm ? p.rep_perm + rb : nullptr, p.ref, lratio,
-                                             acc, od);
+    int od;
+    const double d = sweep_direct<T, G, true>(ch, true, s_h, c, p.Jt, beta, (uint64_t)chain,
+                                              p.rep_lr ? p.rep_lr + rb : nullptr,
+                                              p.rep_perm ? p.rep_perm + rb : nullptr, acc, od);
     E += d;
     ones += od;
     if (p.ordering == 0) {  // simulation.py:309-314, after the transition
@@ -388,7 +507,7 @@ __global__ __launch_bounds__(256) void plain_ais_kernel(AISParams<T> p) {
     }
     bold = bd;
   }
-  ch.store(p.fields + off, p.spins + off, n, p.ptype);
+  ch.store(p.fields + (size_t)chain * n, p.spins + (size_t)chain * n, n, p.ptype);
   if (lane == 0) {
     p.energy[chain] = E;
     p.logw[chain] = lw;
@@ -421,61 +540,55 @@ __global__ __launch_bounds__(256) void energy_f64_kernel(const T* __restrict__ J
   }
 }
 
-int chunks_for(int n) {
-  int ch = 1;
-  while (64 * ch < n) ch *= 2;
-  return ch;
+// Field registers per lane for n spins: a power of two from 4 to 64 with
+// 64 * G >= n.
+int regs_for(int n) {
+  int g = 4;
+  while (64 * g < n) g *= 2;
+  return g;
+}
+
+template <typename T, int G>
+hipError_t launch_pt_shape(const PTParams<T>& p, hipStream_t st) {
+  const int W = min(p.R, kernel_waves<T, G>());
+  hipLaunchKernelGGL((plain_pt_kernel<T, G>), dim3(p.reads), dim3(64 * W), 0, st, p);
+  return hipGetLastError();
+}
+
+template <typename T, int G>
+hipError_t launch_ais_shape(const AISParams<T>& p, hipStream_t st) {
+  const int W = kernel_waves<T, G>();
+  hipLaunchKernelGGL((plain_ais_kernel<T, G>), dim3((p.chains + W - 1) / W), dim3(64 * W), 0, st, p);
+  return hipGetLastError();
 }
 
 }  // namespace
 
 template <typename T>
 hipError_t launch_plain_pt(const PTParams<T>& p, hipStream_t st) {
-  if (p.n < 1 || p.n > kPlainMaxN || p.R < 1 || p.R > kMaxR || p.reads < 1) return hipErrorInvalidValue;
-  const int ch = chunks_for(p.n);
-  dim3 grid(p.reads);
-#define PYSA_LAUNCH(CHV)                                                                   \
-  case CHV:                                                                                \
-    hipLaunchKernelGGL((plain_pt_kernel<T, CHV>), grid, dim3(64 * min(p.R, waves_for<T, CHV>())), \
-                       0, st, p);                                                          \
-    break;
-  switch (ch) {
-    PYSA_LAUNCH(1)
-    PYSA_LAUNCH(2)
-    PYSA_LAUNCH(4)
-    PYSA_LAUNCH(8)
-    PYSA_LAUNCH(16)
-    PYSA_LAUNCH(32)
-    PYSA_LAUNCH(64)
-    default:
-      return hipErrorInvalidValue;
+  if (p.n < 1 || p.n > kPlainMaxN || p.R < 1 || p.R > kMaxR || p.reads < 1)
+    return hipErrorInvalidValue;
+  switch (regs_for(p.n)) {
+    case 4: return launch_pt_shape<T, 4>(p, st);
+    case 8: return launch_pt_shape<T, 8>(p, st);
+    case 16: return launch_pt_shape<T, 16>(p, st);
+    case 32: return launch_pt_shape<T, 32>(p, st);
+    case 64: return launch_pt_shape<T, 64>(p, st);
+    default: return hipErrorInvalidValue;
   }
-#undef PYSA_LAUNCH
-  return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_plain_ais(const AISParams<T>& p, hipStream_t st) {
   if (p.n < 1 || p.n > kPlainMaxN || p.chains < 1) return hipErrorInvalidValue;
-  const int ch = chunks_for(p.n);
-  const int W = 4;
-  dim3 grid((p.chains + W - 1) / W), block(64 * W);
-#define PYSA_LAUNCH(CHV) \
-  case CHV:              \
-    hipLaunchKernelGGL((plain_ais_kernel<T, CHV>), grid, block, 0, st, p); break;
-  switch (ch) {
-    PYSA_LAUNCH(1)
-    PYSA_LAUNCH(2)
-    PYSA_LAUNCH(4)
-    PYSA_LAUNCH(8)
-    PYSA_LAUNCH(16)
-    PYSA_LAUNCH(32)
-    PYSA_LAUNCH(64)
-    default:
-      return hipErrorInvalidValue;
+  switch (regs_for(p.n)) {
+    case 4: return launch_ais_shape<T, 4>(p, st);
+    case 8: return launch_ais_shape<T, 8>(p, st);
+    case 16: return launch_ais_shape<T, 16>(p, st);
+    case 32: return launch_ais_shape<T, 32>(p, st);
+    case 64: return launch_ais_shape<T, 64>(p, st);
+    default: return hipErrorInvalidValue;
   }
-#undef PYSA_LAUNCH
-  return hipGetLastError();
 }
 
 template <typename T>
@@ -496,4 +609,5 @@ template hipError_t launch_energy_f64<float>(const float*, const float*, const i
                                              long long, hipStream_t);
 template hipError_t launch_energy_f64<double>(const double*, const double*, const int8_t*, double*,
                                               int, long long, hipStream_t);
+
 }  // namespace pysa
