@@ -2,7 +2,8 @@
 
 Workload (one "step"): the planted Wishart Ising instance, N=1024, alpha=0.5
 (M=512), float32 couplings, h=0; 512 reads x 32 PT replicas x 5000 sweeps,
-temperatures geometric 0.3 -> 1.5 (PySA's defaults), random update order.
+temperatures geometric 0.3 -> 1.5 (PySA's defaults), sequential update
+order (the order both engine forms run with the same semantics).
 All inputs (couplings, initial states, ladders) are resident in HBM before
 the timed region; a step re-copies the initial states device-to-device and
 runs every read of the batch.
@@ -106,7 +107,7 @@ class Workload:
 def run_form(wl, form_name, steps, warmup, seed):
     L = wl.lib
     form = {"plain": L.FORM_PLAIN, "blocked": L.FORM_BLOCKED}[form_name]
-    order = L.ORDER["random"] if form_name == "plain" else L.ORDER["random_shared"]
+    order = L.ORDER["sequential"]
     for w in range(warmup):
         wl.step(form, order, seed + 7919 * (w + 1))
         log(f"[bench] {form_name}: warmup step {w + 1}/{warmup} done")
@@ -163,18 +164,18 @@ def cpu_baseline(target_s=12.0, threads=None):
     if threads is None:
         threads = min(16, len(os.sched_getaffinity(0)))
     oracle.load()
-    oracle.metropolis(J, h, 0, betas, st, en, 5, order=1, seed=4, nthreads=threads)  # warm
+    oracle.metropolis(J, h, 0, betas, st, en, 5, order=0, seed=4, nthreads=threads)  # warm
     t0 = time.perf_counter()
-    oracle.metropolis(J, h, 0, betas, st, en, 40, order=1, seed=5, nthreads=threads)
+    oracle.metropolis(J, h, 0, betas, st, en, 40, order=0, seed=5, nthreads=threads)
     dt = time.perf_counter() - t0
     sweeps = max(40, int(40 * target_s / max(dt, 1e-3)))
     t0 = time.perf_counter()
-    oracle.metropolis(J, h, 0, betas, st, en, sweeps, order=1, seed=6, nthreads=threads)
+    oracle.metropolis(J, h, 0, betas, st, en, sweeps, order=0, seed=6, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": n * R * sweeps / dt, "unit": "spin_updates/s", "cores": threads,
             "kind": "port",
             "sample": f"1 read x {R} replicas x {sweeps} sweeps of the same N=1024 Wishart "
-                      f"instance ({dt:.1f} s, random order, PT on)"}
+                      f"instance ({dt:.1f} s, sequential order, PT on)"}
 
 
 def main():
@@ -234,7 +235,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"planted Wishart Ising N={n} alpha=0.5 float32, "
                                f"{args.reads} reads x {args.replicas} replicas x {args.sweeps} "
-                               "sweeps, T 0.3->1.5, random order, PT",
+                               "sweeps, T 0.3->1.5, sequential order, PT",
                    "form": head, "parallelism": "single GPU (reads across workgroups)"},
         "roofline": roof,
         "forms": res,

@@ -44,10 +44,13 @@ extern "C" {
 #define PYSA_AMD_EHIP (-2)    /* HIP runtime error */
 #define PYSA_AMD_ENOTSUP (-3) /* valid request this build does not support */
 
-/* Engine forms accepted by pysa_amd_metropolis*. */
+/* Engine forms accepted by pysa_amd_metropolis*.  BLOCKED encodes real
+ * couplings as hi+lo bf16 pieces (integer couplings exactly); BLOCKED_EXACT
+ * uses hi+mid+lo pieces, exact for every float32. */
 #define PYSA_AMD_FORM_PLAIN 0
 #define PYSA_AMD_FORM_BLOCKED 1
 #define PYSA_AMD_FORM_AUTO 2
+#define PYSA_AMD_FORM_BLOCKED_EXACT 3
 
 /* Message describing the last failure on this thread ("" if none). */
 const char* pysa_amd_last_error(void);
@@ -82,7 +85,9 @@ int pysa_amd_problem_destroy(int64_t handle);
  *   beta_sort        R slot indices sorted by ascending beta (log_omega only)
  *   seed             Philox key; sweep_offset shifts the sweep counter so a
  *                    resumed run continues the same stream
- *   form, block_b    engine form and blocked-form block size (0 = autotune)
+ *   form             PYSA_AMD_FORM_*; AUTO picks BLOCKED where it applies
+ *                    (float32, sequential order, no replay, R <= 64, n <= 2048)
+ *   block_b          reserved (blocked-form block size; 0 = default 32)
  *   replay_log_r     NULL, or reads*S*R*n log-uniforms indexed
  *                    [read][sweep][replica][position] (replay mode)
  *   replay_perm      NULL, or reads*S*R*n visit orders [read][sweep][replica][i]

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpysa_amd.so")
 
-FORM_PLAIN, FORM_BLOCKED, FORM_AUTO = 0, 1, 2
+FORM_PLAIN, FORM_BLOCKED, FORM_AUTO, FORM_BLOCKED_EXACT = 0, 1, 2, 3
 ORDER = {"sequential": 0, "random": 1, "random_shared": 2}
 
 _lib = None
@@ -182,4 +182,4 @@ def blocked_available():
 
 
 # The blocked form is selected by form='auto' only once it is built.
-BLOCKED_READY = False
+BLOCKED_READY = True

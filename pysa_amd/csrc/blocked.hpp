@@ -7,26 +7,33 @@
 
 namespace pysa {
 
+// Largest problem the blocked form holds (spin bits of 64 replicas in VGPRs).
+constexpr int kBlockedMaxN = 2048;
+
 // Packed couplings and scratch of the blocked form, owned by a problem.
 struct BlockedProblem {
-  void* packed = nullptr;  // MFMA-ready coupling tiles
+  void* packed = nullptr;  // MFMA A-operand fragments of J
   size_t packed_bytes = 0;
-  int packed_b = 0, packed_split = 0;
+  int packed_kind = -1;  // 0 int8, 1..3 bf16 pieces
+  int exact_kind = -1;   // smallest exact encoding of this J
+  int last_kind = -1;    // encoding used by the last run
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
 };
 
 void blocked_release(BlockedProblem& b);
 
-// Whether the blocked form can run this request (auto form selection).
+// Whether the blocked form can run this request.
 bool blocked_eligible(int n, int dtype, int R, int reads, int order, bool replay);
 
-int run_blocked(BlockedProblem& b, const void* dJ, const void* dh, int n, int ptype, int dtype,
-                int reads, int R, int S, int order, int use_pt, int logw, int best_sem, int block_b,
-                uint64_t seed, uint64_t sweep_offset, const void* betas, const int32_t* bsort,
-                const double* rep_lr, const double* rep_pt, int8_t* states, void* energies,
-                int64_t* beta_idx, int8_t* best_state, void* best_energy, int32_t* best_sweeps,
-                double* log_omegas, uint64_t* accepted, hipStream_t st, double* ms,
-                std::string* why);
+// pieces: 0 = exact where possible else hi+lo bf16; 2 or 3 = that many bf16
+// pieces (3 is exact for every f32).
+int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* dh, int n,
+                int ptype, int dtype, int reads, int R, int S, int order, int use_pt, int logw,
+                int best_sem, int pieces, uint64_t seed, uint64_t sweep_offset, const void* betas,
+                const int32_t* bsort, const double* rep_lr, const double* rep_pt, int8_t* states,
+                void* energies, int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                int32_t* best_sweeps, double* log_omegas, uint64_t* accepted, hipStream_t st,
+                double* ms, std::string* why);
 
 }  // namespace pysa

@@ -132,7 +132,7 @@ int check_run(const Problem* P, const RunArgs& a) {
     return fail(PYSA_AMD_EINVAL, "log_omega needs beta_sort and log_omegas");
   if (a.order < 0 || a.order > 2) return fail(PYSA_AMD_EINVAL, "order must be 0, 1 or 2");
   if (a.best_sem < 0 || a.best_sem > 1) return fail(PYSA_AMD_EINVAL, "best_semantics must be 0 or 1");
-  if (P->n > pysa::kPlainMaxN)
+  if (a.form == PYSA_AMD_FORM_PLAIN && P->n > pysa::kPlainMaxN)
     return fail(PYSA_AMD_ENOTSUP, "n above the plain kernel's register budget (4096)");
   return 0;
 }
@@ -209,11 +209,17 @@ int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
 template <typename T>
 int run_device(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
   int form = a.form;
-  if (form == PYSA_AMD_FORM_AUTO) form = pysa::blocked_eligible(P->n, P->dtype, a.R, a.reads, a.order, a.rep_lr != nullptr) ? PYSA_AMD_FORM_BLOCKED : PYSA_AMD_FORM_PLAIN;
-  if (form == PYSA_AMD_FORM_BLOCKED) {
+  if (form == PYSA_AMD_FORM_AUTO)
+    form = pysa::blocked_eligible(P->n, P->dtype, a.R, a.reads, a.order,
+                                  a.rep_lr != nullptr || a.rep_pt != nullptr || a.rep_perm != nullptr)
+               ? PYSA_AMD_FORM_BLOCKED
+               : PYSA_AMD_FORM_PLAIN;
+  if (form == PYSA_AMD_FORM_BLOCKED || form == PYSA_AMD_FORM_BLOCKED_EXACT) {
+    if (a.rep_perm) return fail(PYSA_AMD_ENOTSUP, "the blocked form does not replay permutations");
     std::string why;
-    int rc = pysa::run_blocked(P->blocked, P->dJ, P->dh, P->n, P->ptype, P->dtype, a.reads, a.R,
-                               a.S, a.order, a.use_pt, a.logw, a.best_sem, a.block_b, a.seed,
+    int rc = pysa::run_blocked(P->blocked, P->dJ, P->dJt, P->dh, P->n, P->ptype, P->dtype, a.reads,
+                               a.R, a.S, a.order, a.use_pt, a.logw, a.best_sem,
+                               form == PYSA_AMD_FORM_BLOCKED_EXACT ? 3 : 0, a.seed,
                                a.sweep_offset, a.betas, a.bsort, a.rep_lr, a.rep_pt, a.states,
                                a.energies, a.beta_idx, a.best_state, a.best_energy, a.best_sweeps,
                                a.log_omegas, a.accepted, st, ms, &why);

@@ -35,7 +35,8 @@ Vector = List[float]
 Matrix = List[List[float]]
 State = List[float]
 
-_BLOCKED_MIN_N = 64
+_BLOCKED_MIN_N = 32
+_BLOCKED_MAX_N = 2048
 
 
 def _tqdm(it, disable):
@@ -242,7 +243,8 @@ class Solver(object):
         if engine_stats is not None:
             upd = float(N) * num_sweeps * R * num_reads
             engine_stats.update({
-                'form': 'blocked' if form_code == _lib.FORM_BLOCKED else 'plain',
+                'form': {_lib.FORM_BLOCKED: 'blocked', _lib.FORM_BLOCKED_EXACT: 'blocked_exact'}.get(
+                    form_code, 'plain'),
                 'order': order, 'seed': int(seed), 'kernel_ms': kms.value, 'wall_s': wall,
                 'accepted': accepted.copy(), 'updates': upd,
                 'updates_per_s': upd / (kms.value * 1e-3) if kms.value > 0 else float('nan'),
@@ -360,16 +362,24 @@ class Solver(object):
         return _init
 
     def _resolve_form(self, form, update_strategy, ft, R, replay):
-        if form not in ('auto', 'plain', 'blocked'):
+        """Engine form: 'plain' runs every request; 'blocked' / 'blocked_exact'
+        (the MFMA form) need float32, sequential order, Philox randomness,
+        at most 64 replicas and n <= 2048; 'auto' takes the blocked form when
+        it applies."""
+        if form not in ('auto', 'plain', 'blocked', 'blocked_exact'):
             raise ValueError(f"form='{form}' not recognized.")
+        ok = (ft == np.float32 and _BLOCKED_MIN_N <= self.n_vars <= _BLOCKED_MAX_N and
+              R <= 64 and replay is None and update_strategy == 'sequential' and
+              _lib.blocked_available())
         if form == 'auto':
-            form = 'blocked' if (ft == np.float32 and self.n_vars >= _BLOCKED_MIN_N and
-                                 replay is None and _lib.blocked_available()) else 'plain'
-        if form == 'blocked':
-            if ft != np.float32:
-                raise ValueError("form='blocked' requires float_type='float32'.")
-            order = _lib.ORDER['sequential' if update_strategy == 'sequential' else 'random_shared']
-            return _lib.FORM_BLOCKED, order
+            form = 'blocked' if ok else 'plain'
+        if form in ('blocked', 'blocked_exact'):
+            if not ok:
+                raise ValueError(
+                    f"form='{form}' needs float_type='float32', update_strategy='sequential', "
+                    f"{_BLOCKED_MIN_N} <= n <= {_BLOCKED_MAX_N}, num_replicas <= 64 and no replay.")
+            code = _lib.FORM_BLOCKED if form == 'blocked' else _lib.FORM_BLOCKED_EXACT
+            return code, _lib.ORDER['sequential']
         return _lib.FORM_PLAIN, _lib.ORDER[update_strategy]
 
     def annealed_importance(self,
