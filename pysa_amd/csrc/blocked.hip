@@ -27,10 +27,14 @@
 // certified single-precision fast path with an exact f64 fallback, so with
 // exact couplings the trajectories equal the plain form's bit for bit.
 //
-// Layout of a wave: 64 replicas = floor(64/R) reads of R replicas; replicas
-// q and q+32 form MFMA column tiles 0 and 1.  One launch runs all sweeps;
-// parallel tempering, best tracking and log-omega run per read inside the
-// wave after every sweep.
+// Layout: a workgroup owns a group of 64 replicas = floor(64/R) reads of R
+// replicas (replicas q and q+32 form MFMA column tiles 0 and 1), split over
+// KW = 1, 2 or 4 waves so that small batches still occupy every SIMD: each
+// wave holds the spins of one K range of the problem for all 64 replicas and
+// multiplies that range; the partial block fields are summed in LDS, where
+// each wave then runs the proposals of 64/KW replicas.  One launch runs all
+// sweeps; parallel tempering, best tracking and log-omega run per read
+// inside the workgroup after every sweep.
 #include "blocked.hpp"
 
 #include <math.h>
@@ -51,9 +55,7 @@ typedef __attribute__((ext_vector_type(4))) int i32x4;
 typedef __attribute__((ext_vector_type(16))) int i32x16;
 typedef __attribute__((ext_vector_type(4))) unsigned u32x4v;
 
-constexpr int kB = 32;            // block size (rows of one MFMA tile)
-constexpr int kWavesPerGroup = 4; // waves per workgroup
-constexpr int kLdsPerWave = 2 * kB * 33 * 4 + 64 * 4 * 4 + 64 * 8;  // transpose + PT scratch
+constexpr int kB = 32;  // block size (rows of one MFMA tile)
 
 // ---------------------------------------------------------------- packing --
 __host__ __device__ inline uint16_t bf16_rn(float f) {
@@ -174,6 +176,23 @@ struct Bits {
   V w;
 };
 
+// Value of sub-lane `a` of each group of LPR consecutive lanes, broadcast to
+// the whole group (DPP quad permute; LPR divides 4).
+template <int LPR>
+__device__ __forceinline__ float group_bcast(float v, int a) {
+  if (LPR == 1) return v;
+  const int ctrl = (LPR == 4) ? (a * 0x55) : (a | (a << 2) | ((2 + a) << 4) | ((2 + a) << 6));
+  int r;
+  switch (ctrl) {  // the DPP control must be a compile-time constant
+#define PYSA_DPP(C) \
+  case C: r = __builtin_amdgcn_mov_dpp(__float_as_int(v), C, 0xF, 0xF, false); break;
+    PYSA_DPP(0x00) PYSA_DPP(0x55) PYSA_DPP(0xAA) PYSA_DPP(0xFF) PYSA_DPP(0xA0) PYSA_DPP(0xF5)
+#undef PYSA_DPP
+    default: r = __float_as_int(v);
+  }
+  return __int_as_float(r);
+}
+
 template <int NW>
 __device__ __forceinline__ uint32_t get_half(const Bits<NW>& s, int c) {
   return (s.w[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
@@ -185,7 +204,8 @@ __device__ __forceinline__ void set_half(Bits<NW>& s, int c, uint32_t v) {
 }
 
 struct BParams {
-  int n, nblk, ptype, R, reads, S, rpw, np, use_pt, logw, best_sem, refresh;
+  int n, nblk, ptype, R, reads, S, rpw, np, use_pt, logw, best_sem;
+  int kb;  // row blocks of the K range each wave of a group owns
   uint32_t k0, k1;
   uint64_t sweep_offset;
   const void* frag;
@@ -206,14 +226,31 @@ struct BParams {
   unsigned long long* accepted;
 };
 
-// Write the spins of replica `q` of this wave (held in tile q/32, lanes
-// q%32 and q%32+32) to dst[0..n): called by both lanes of the pair.
+// Workgroup-shared state of one replica group (64 replicas, KW waves).
+template <int KW>
+struct GroupLds {
+  float part[KW][2][32 * 33];  // partial block fields: [wave][tile][replica col * 33 + row]
+  uint32_t bb[2][64];          // block spin bits per replica (double-buffered by block parity)
+  float jb[kB * kB];           // jb[i*32 + k] = J^T[pos0+i][pos0+k] = J[pos0+k][pos0+i]
+  float hb[kB];
+  float E[64];
+  float rb[64];   // beta of each replica
+  int bi[64];     // slot -> replica within its read, at g*R + slot
+  int flag[64];   // per-replica improvement flags
+  int src[64];    // per-read source replica of a new global best (-1: none)
+  double lu[64];  // PT log-uniforms, at g*R + pair
+};
+
+// Store the spins of one operand-role replica for the blocks this wave owns.
 template <int NW, bool I8>
-__device__ __forceinline__ void store_state(const Bits<NW>& s, int nblk, int n, int ptype, int8_t* dst) {
+__device__ __forceinline__ void store_range(const Bits<NW>& s, int c0, int kb, int nblk, int n,
+                                            int ptype, int8_t* dst) {
   const int h = (threadIdx.x & 63) >> 5;
   const int8_t dn = ptype == 0 ? int8_t(-1) : int8_t(0);
-  for (int c = 0; c < nblk; ++c) {
-    const uint32_t hv = get_half<NW>(s, c);
+  for (int j = 0; j < kb; ++j) {
+    const int c = c0 + j;
+    if (c >= nblk) break;
+    const uint32_t hv = get_half<NW>(s, j);
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
       const int rel = I8 ? (16 * h + b) : ((b < 8) ? (8 * h + b) : (16 + 8 * h + (b - 8)));
@@ -223,40 +260,42 @@ __device__ __forceinline__ void store_state(const Bits<NW>& s, int nblk, int n, 
   }
 }
 
-template <int NP, bool I8, int NW>
-__global__ __launch_bounds__(64 * kWavesPerGroup) void blocked_pt_kernel(BParams p) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned char* my = lds_raw + (size_t)wave * kLdsPerWave;
-  float* s_tr = reinterpret_cast<float*>(my);                       // 2 tiles x 32 x 33
-  float* s_E = reinterpret_cast<float*>(my + 2 * kB * 33 * 4);       // 64
-  int* s_bi = reinterpret_cast<int*>(s_E + 64);                      // 64: slot -> replica per read
-  float* s_rb = reinterpret_cast<float*>(s_bi + 64);                 // 64: beta of each replica
-  int* s_flag = reinterpret_cast<int*>(s_rb + 64);                   // 64
-  double* s_lu = reinterpret_cast<double*>(s_flag + 64);             // 64: PT log-uniforms
-
-  const int R = p.R, n = p.n, nblk = p.nblk;
-  const int gw = blockIdx.x * kWavesPerGroup + wave;  // global wave
-  const int read0 = gw * p.rpw;
-  // In-block role: lane = replica slot q of this wave.
-  const int q = lane;
-  const int g = q / R, rr = q - g * R;  // read within the wave, replica within the read
-  const int read = read0 + g;
-  const bool act = (g < p.rpw) && (read < p.reads);
+// One workgroup = one group of 64 replicas (floor(64/R) reads) split over KW
+// waves.  Wave w owns the spins of blocks [w*kb, (w+1)*kb) for all 64
+// replicas (MFMA operand layout) and computes the partial block fields over
+// that K range; the partials meet in LDS, where wave w then runs the block's
+// proposals for replicas [w*64/KW, (w+1)*64/KW), one lane per replica.
+template <int NP, bool I8, int NW, int KW>
+__global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
+  __shared__ GroupLds<KW> L;
+  constexpr int Q = 64 / KW;  // replicas per wave in the proposal phase
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tid = threadIdx.x;
+  const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
+  const int read0 = blockIdx.x * p.rpw;
   const int h = lane >> 5;
-  // MFMA-operand role: lane holds tile-0 replica lane%32 and tile-1 replica 32+lane%32.
+  // Operand role: tile-0 replica qa and tile-1 replica qb of the group.
   const int qa = lane & 31, qb = 32 + (lane & 31);
   const int ga = qa / R, gb = qb / R;
   const bool acta = (ga < p.rpw) && (read0 + ga < p.reads);
   const bool actb = (gb < p.rpw) && (read0 + gb < p.reads);
-  const size_t base_a = ((size_t)(read0 + ga) * R + (qa - ga * R)) * n;
-  const size_t base_b = ((size_t)(read0 + gb) * R + (qb - gb * R)) * n;
+  const size_t base_a = acta ? ((size_t)(read0 + ga) * R + (qa - ga * R)) * n : 0;
+  const size_t base_b = actb ? ((size_t)(read0 + gb) * R + (qb - gb * R)) * n : 0;
+  const int c0 = w * kb;  // first block of this wave's K range
+  // Proposal role: the LPR lanes a = 0..LPR-1 of group lane/LPR hold replica
+  // q of the group, sub-lane a owning block rows [a*RPL, (a+1)*RPL).
+  constexpr int LPR = KW, RPL = kB / KW;
+  const int a = lane % LPR;
+  const int q = w * Q + lane / LPR;
+  const int g = q / R, rr = q - g * R;
+  const int read = read0 + g;
+  const bool act = (g < p.rpw) && (read < p.reads);
 
-  // Spin bits in operand layout.
   Bits<NW> sa, sb;
   sa.w = 0;
   sb.w = 0;
-  for (int c = 0; c < nblk; ++c) {
+  for (int j = 0; j < kb; ++j) {
+    const int c = c0 + j;
+    if (c >= nblk) break;
     uint32_t va = 0, vb = 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
@@ -267,44 +306,59 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void blocked_pt_kernel(BParams
         if (actb && p.spins[base_b + pos] == 1) vb |= 1u << b;
       }
     }
-    set_half<NW>(sa, c, va);
-    set_half<NW>(sb, c, vb);
+    set_half<NW>(sa, j, va);
+    set_half<NW>(sb, j, vb);
   }
   float E = act ? p.energies[(size_t)read * R + rr] : 0.f;
-  // Slot map and betas (sa.py:325, simulation.py:151-152).
-  if (act) {
-    s_bi[q] = rr;  // beta_idx = arange(R) for each read
-    s_rb[q] = p.betas[rr];
+  if (act && a == 0) {
+    L.bi[q] = rr;  // beta_idx = arange(R) (sa.py:325)
+    L.rb[q] = p.betas[rr];
   }
   float bestE = E;
-  if (p.best_sem == 0 && act) {
+  if (p.best_sem == 0 && act && a == 0) {
     p.best_rep_e[(size_t)read * R + rr] = E;
     p.best_rep_sw[(size_t)read * R + rr] = 0;
   }
-  float gE = 0.f;
+  float gE = E;  // simulation.py:205 (meaningful on read leaders, rr == 0)
   int gsw = 0, gimp = 0;
-  if (act && rr == 0) gE = E;  // simulation.py:205 (read leader)
   const uint64_t cid = (uint64_t)read * R + rr;
   unsigned long long acc = 0;
   __syncthreads();
 
   for (int s = 0; s < p.S; ++s) {
     const uint64_t sw = (uint64_t)s + p.sweep_offset;
-    const float beta = act ? s_rb[q] : 0.f;
+    const float beta = act ? L.rb[q] : 0.f;
     const double bd = (double)beta;
     double dsum = 0.0;
     for (int c = 0; c < nblk; ++c) {
-      // ---- 1. block fields on the matrix cores
+      const int buf = c & 1, oc = c / kb, pos0 = 32 * c;
+      // ---- phase 1a: stage J[blk][blk] (as J^T rows) and h[blk]
+      for (int e = tid; e < kB * kB; e += 64 * KW) {
+        const int i = e >> 5, k = e & 31;
+        L.jb[i * kB + k] = (pos0 + i < n && pos0 + k < n) ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
+      }
+      if (tid < kB) L.hb[tid] = (pos0 + tid < n) ? p.h[pos0 + tid] : 0.f;
+      // ---- phase 1b: the owner publishes the block's spins of all 64 replicas
+      if (w == oc) {
+        const int j = c - c0;
+        const uint32_t mine = (lane < 32) ? get_half<NW>(sa, j) : get_half<NW>(sb, j);
+        const uint32_t give = (lane < 32) ? get_half<NW>(sb, j) : get_half<NW>(sa, j);
+        const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
+        L.bb[buf][lane] = (lane < 32) ? block_of<I8>(mine, got) : block_of<I8>(got, mine);
+      }
+      // ---- phase 1c: partial block fields over this wave's K range
       f32x16 acc0 = {}, acc1 = {};
       i32x16 iacc0 = {}, iacc1 = {};
-      for (int cp = 0; cp < nblk; ++cp) {
-        const uint32_t ha = get_half<NW>(sa, cp), hb = get_half<NW>(sb, cp);
-        if (I8) {
-          const i32x4* fr = reinterpret_cast<const i32x4*>(p.frag) + ((size_t)c * nblk + cp) * 64 + lane;
-          const i32x4 a = *fr;
+      const int jn = min(kb, nblk - c0);
+      if (I8) {
+        const i32x4* base = reinterpret_cast<const i32x4*>(p.frag) + ((size_t)c * nblk + c0) * 64 + lane;
+        i32x4 a_nx = (jn > 0) ? base[0] : i32x4{0, 0, 0, 0};
+        for (int j = 0; j < jn; ++j) {
+          const i32x4 a = a_nx;
+          if (j + 1 < jn) a_nx = base[(size_t)(j + 1) * 64];
           uint32_t da[4], db[4];
-          spins_i8(ha, p.ptype, da);
-          spins_i8(hb, p.ptype, db);
+          spins_i8(get_half<NW>(sa, j), p.ptype, da);
+          spins_i8(get_half<NW>(sb, j), p.ptype, db);
           i32x4 ba, bb;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
@@ -313,14 +367,33 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void blocked_pt_kernel(BParams
           }
           iacc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ba, iacc0, 0, 0, 0);
           iacc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bb, iacc1, 0, 0, 0);
-        } else {
+        }
+      } else {
+        const bf16x8* base =
+            reinterpret_cast<const bf16x8*>(p.frag) + ((size_t)c * (2 * nblk) + 2 * c0) * NP * 64 + lane;
+        bf16x8 a_nx[2][NP];
+        if (jn > 0) {
+#pragma unroll
+          for (int st = 0; st < 2; ++st)
+#pragma unroll
+            for (int pc = 0; pc < NP; ++pc) a_nx[st][pc] = base[(st * NP + pc) * 64];
+        }
+        for (int j = 0; j < jn; ++j) {
+          bf16x8 a[2][NP];
+#pragma unroll
+          for (int st = 0; st < 2; ++st)
+#pragma unroll
+            for (int pc = 0; pc < NP; ++pc) a[st][pc] = a_nx[st][pc];
+          if (j + 1 < jn) {
+#pragma unroll
+            for (int st = 0; st < 2; ++st)
+#pragma unroll
+              for (int pc = 0; pc < NP; ++pc)
+                a_nx[st][pc] = base[((size_t)(2 * (j + 1) + st) * NP + pc) * 64];
+          }
+          const uint32_t ha = get_half<NW>(sa, j), hb = get_half<NW>(sb, j);
 #pragma unroll
           for (int st = 0; st < 2; ++st) {
-            const bf16x8* fr = reinterpret_cast<const bf16x8*>(p.frag) +
-                               (((size_t)c * (2 * nblk) + 2 * cp + st) * NP) * 64 + lane;
-            bf16x8 a[NP];
-#pragma unroll
-            for (int pc = 0; pc < NP; ++pc) a[pc] = fr[pc * 64];
             uint32_t da[4], db[4];
             spins_bf16((ha >> (8 * st)) & 0xFFu, p.ptype, da);
             spins_bf16((hb >> (8 * st)) & 0xFFu, p.ptype, db);
@@ -328,104 +401,119 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void blocked_pt_kernel(BParams
             const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bb = __builtin_bit_cast(bf16x8, ub);
 #pragma unroll
             for (int pc = 0; pc < NP; ++pc) {
-              acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pc], ba, acc0, 0, 0, 0);
-              acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pc], bb, acc1, 0, 0, 0);
+              acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[st][pc], ba, acc0, 0, 0, 0);
+              acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[st][pc], bb, acc1, 0, 0, 0);
             }
           }
         }
       }
-      // ---- 2. transpose through LDS: lane q receives the 32 fields of replica q
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int i = 8 * (v >> 2) + 4 * h + (v & 3);
-        const float f0 = I8 ? (float)iacc0[v] : acc0[v];
-        const float f1 = I8 ? (float)iacc1[v] : acc1[v];
-        s_tr[(lane & 31) * 33 + i] = f0;
-        s_tr[(32 + (lane & 31)) * 33 + i] = f1;
+        L.part[w][0][(lane & 31) * 33 + i] = I8 ? (float)iacc0[v] : acc0[v];
+        L.part[w][1][(lane & 31) * 33 + i] = I8 ? (float)iacc1[v] : acc1[v];
       }
-      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-      __builtin_amdgcn_wave_barrier();
-      float f[kB];
+      __syncthreads();
+      // ---- phase 2: the block's proposals in sequential order.  Replica q's
+      // LPR lanes each hold RPL of the 32 block fields; the lane owning row
+      // k decides, broadcasts dS, and every lane updates its rows from row k
+      // of J^T[blk][blk].
+      {
+        const int tq = q >> 5, jq = q & 31, r0 = a * RPL;
+        float f[RPL], hv[RPL];
 #pragma unroll
-      for (int i = 0; i < kB; ++i) f[i] = s_tr[q * 33 + i];
-      // Block spins of replica q: halves live in lanes q%32 (h=0) and q%32+32 (h=1)
-      // of tile q/32.
-      const uint32_t mine = (lane < 32) ? get_half<NW>(sa, c) : get_half<NW>(sb, c);
-      const uint32_t give = (lane < 32) ? get_half<NW>(sb, c) : get_half<NW>(sa, c);
-      const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
-      uint32_t bits = (lane < 32) ? block_of<I8>(mine, got) : block_of<I8>(got, mine);
-      // ---- 3. the block's proposals (sequential order).  Flipping spin j
-      // adds J[k][j] dS to the field of spin k: row j of J^T.
-      const int pos0 = 32 * c;
-      const float* Jb = p.Jt + (size_t)pos0 * n + pos0;
-      uint32_t wrd[4] = {0, 0, 0, 0};
+        for (int rl = 0; rl < RPL; ++rl) {
+          float t = 0.f;
 #pragma unroll
-      for (int i = 0; i < kB; ++i) {
-        const int pos = pos0 + i;
-        if (pos >= n) break;
-        if ((i & 3) == 0) {
-          const u32x4 blk = sweep_block(cid, sw, (uint32_t)pos, p.k0, p.k1);
-          wrd[0] = blk.x;
-          wrd[1] = blk.y;
-          wrd[2] = blk.z;
-          wrd[3] = blk.w;
+          for (int ww = 0; ww < KW; ++ww) t += L.part[ww][tq][jq * 33 + r0 + rl];
+          f[rl] = t;
+          hv[rl] = L.hb[r0 + rl];
         }
-        const bool up = (bits >> i) & 1u;
-        const float fh = f[i] + p.h[pos];
-        // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
-        const float dn32 = (p.ptype == 0) ? (up ? 2.f : -2.f) * fh : (up ? fh : -fh);
-        bool take = dn32 >= 0.f;
-        if (!take) take = log_less(wrd[i & 3], beta * dn32, bd * (double)dn32);
-        take = take && act;
-        bits ^= (uint32_t)take << i;
-        dsum += take ? -(double)dn32 : 0.0;
-        acc += take ? 1u : 0u;
-        const float dS = take ? ((p.ptype == 0) ? (up ? -2.f : 2.f) : (up ? -1.f : 1.f)) : 0.f;
-        const float* jr = Jb + (size_t)i * n;
+        const uint32_t all = L.bb[buf][q];
+        uint32_t mb = (RPL == 32) ? all : ((all >> r0) & ((1u << RPL) - 1u));
+        uint32_t uw[RPL];
 #pragma unroll
-        for (int k = i + 1; k < kB; ++k)
-          if (pos0 + k < n) f[k] += jr[k] * dS;
+        for (int m = 0; m < RPL / 4; ++m) {
+          const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + r0 + 4 * m), p.k0, p.k1);
+          uw[4 * m] = blk.x;
+          uw[4 * m + 1] = blk.y;
+          uw[4 * m + 2] = blk.z;
+          uw[4 * m + 3] = blk.w;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+          if (pos0 + k >= n) break;
+          const int ak = k / RPL, kl = k % RPL;
+          const bool up = (mb >> kl) & 1u;
+          const float fh = f[kl] + hv[kl];
+          // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
+          const float dn32 = (p.ptype == 0) ? (up ? 2.f : -2.f) * fh : (up ? fh : -fh);
+          bool take = dn32 >= 0.f;
+          if (!take && a == ak) take = log_less(uw[kl], beta * dn32, bd * (double)dn32);
+          take = take && act && (a == ak);
+          mb ^= (uint32_t)take << kl;
+          dsum += take ? -(double)dn32 : 0.0;
+          acc += take ? 1u : 0u;
+          const float dS_own = take ? ((p.ptype == 0) ? (up ? -2.f : 2.f) : (up ? -1.f : 1.f)) : 0.f;
+          const float dS = group_bcast<LPR>(dS_own, ak);
+          const float4* jr = reinterpret_cast<const float4*>(&L.jb[k * kB + r0]);
+#pragma unroll
+          for (int m = 0; m < RPL / 4; ++m) {
+            const float4 j4 = jr[m];
+            f[4 * m] += j4.x * dS;
+            f[4 * m + 1] += j4.y * dS;
+            f[4 * m + 2] += j4.z * dS;
+            f[4 * m + 3] += j4.w * dS;
+          }
+        }
+        if (RPL == 32)
+          L.bb[buf][q] = mb;
+        else if (RPL == 16)
+          reinterpret_cast<uint16_t*>(L.bb[buf])[q * 2 + a] = (uint16_t)mb;
+        else
+          reinterpret_cast<uint8_t*>(L.bb[buf])[q * 4 + a] = (uint8_t)mb;
       }
-      // ---- 4. new block spins back into operand layout
-      const uint32_t h0 = half_of<I8>(bits, 0), h1 = half_of<I8>(bits, 1);
-      // lane l keeps half h=l/32 of tile-0 replica l%32 and tile-1 replica 32+l%32
-      const uint32_t other = (uint32_t)__shfl_xor((int)((lane < 32) ? h1 : h0), 32, 64);
-      if (lane < 32) {
-        set_half<NW>(sa, c, h0);     // tile 0, replica lane, half 0
-        set_half<NW>(sb, c, other);  // tile 1, replica lane+32, half 0 (lane+32's h0)
-      } else {
-        set_half<NW>(sa, c, other);  // tile 0, replica lane-32, half 1 (lane-32's h1)
-        set_half<NW>(sb, c, h1);     // tile 1, replica lane, half 1
+      __syncthreads();
+      // ---- phase 3: the owner takes the new spins back into operand layout
+      if (w == oc) {
+        const int j = c - c0;
+        set_half<NW>(sa, j, half_of<I8>(L.bb[buf][qa], h));
+        set_half<NW>(sb, j, half_of<I8>(L.bb[buf][qb], h));
       }
     }
-    // ---- end of sweep: energy (simulation.py:157-158), best, PT, log-omega
+    // ---- end of sweep: energy (simulation.py:157-158) and best states
+#pragma unroll
+    for (int o = 1; o < LPR; o <<= 1) dsum += __shfl_xor(dsum, o, 64);
     E = (float)((double)E + dsum);
+    if (act && a == 0) L.E[q] = E;
     if (p.best_sem == 0) {  // simulation.py:161-164
       const bool imp = act && (E < bestE);
       if (imp) {
         bestE = E;
-        p.best_rep_e[(size_t)read * R + rr] = E;
-        p.best_rep_sw[(size_t)read * R + rr] = s;
+        if (a == 0) {
+          p.best_rep_e[(size_t)read * R + rr] = E;
+          p.best_rep_sw[(size_t)read * R + rr] = s;
+        }
       }
-      const bool impa = __shfl((int)imp, qa, 64) != 0, impb = __shfl((int)imp, qb, 64) != 0;
-      if (impa) store_state<NW, I8>(sa, nblk, n, p.ptype, p.best_rep + base_a);
-      if (impb) store_state<NW, I8>(sb, nblk, n, p.ptype, p.best_rep + base_b);
+      if (a == 0) L.flag[q] = imp ? 1 : 0;
     }
-    s_E[q] = E;
-    // PT log-uniforms, one lane per pair of this read (drawn ahead of the walk).
-    if (p.use_pt && act && rr < R - 1)
-      s_lu[q] = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)rr, p.k0, p.k1),
+    // PT log-uniforms, one lane per pair of each read (drawn ahead of the walk).
+    if (p.use_pt && act && a == 0 && rr < R - 1)
+      L.lu[q] = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)rr, p.k0, p.k1),
                                   (uint32_t)rr & 3u));
-    __builtin_amdgcn_s_waitcnt(0xc07f);
-    __builtin_amdgcn_wave_barrier();
-    int src = -1;
-    if (act && rr == 0) {  // read leader
+    __syncthreads();
+    if (p.best_sem == 0) {
+      if (acta && L.flag[qa]) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_a);
+      if (actb && L.flag[qb]) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_rep + base_b);
+    }
+    if (act && rr == 0 && a == 0) {  // read leader
       const int b0 = g * R;
+      int src = -1;
       if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
         float cur = gE;
         for (int k = 0; k < R; ++k)
-          if (s_E[b0 + k] < cur) {
-            cur = s_E[b0 + k];
+          if (L.E[b0 + k] < cur) {
+            cur = L.E[b0 + k];
             src = k;
           }
         if (src >= 0) {
@@ -437,65 +525,60 @@ __global__ __launch_bounds__(64 * kWavesPerGroup) void blocked_pt_kernel(BParams
       if (p.use_pt) {  // utils.py:44-58
         for (int k = 0; k < R - 1; ++k) {
           const int a = R - k - 1, b = R - k - 2;
-          const float dd = (s_E[b0 + s_bi[b0 + a]] - s_E[b0 + s_bi[b0 + b]]) *
+          const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
                            (p.betas[a] - p.betas[b]);
-          if (dd >= 0.f || s_lu[b0 + k] < (double)dd) {
-            const int t = s_bi[b0 + a];
-            s_bi[b0 + a] = s_bi[b0 + b];
-            s_bi[b0 + b] = t;
+          if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
+            const int t = L.bi[b0 + a];
+            L.bi[b0 + a] = L.bi[b0 + b];
+            L.bi[b0 + b] = t;
           }
         }
       }
-      for (int k = 0; k < R; ++k) s_rb[b0 + s_bi[b0 + k]] = p.betas[k];
+      for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = p.betas[k];
       if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
         double lw = 0.0, bold = 0.0;
         for (int r = 1; r < R; ++r) {
           const int k = p.bsort[r];
           const float be = p.betas[k];
-          const float en = s_E[b0 + s_bi[b0 + k]];
+          const float en = L.E[b0 + L.bi[b0 + k]];
           const float a = -be * en;
           lw += (double)a - (-bold * (double)en);
           bold = (double)be;
         }
         p.log_omegas[(size_t)read * p.S + s] = lw;
       }
-      s_flag[g] = src;
+      L.src[g] = src;
     }
-    __builtin_amdgcn_s_waitcnt(0xc07f);
-    __builtin_amdgcn_wave_barrier();
-    if (p.best_sem == 1) {
-      // lanes holding replica src of each read copy it to best_state
-      const int fa = (acta ? s_flag[ga] : -1), fb = (actb ? s_flag[gb] : -1);
-      if (fa >= 0 && fa == qa - ga * R)
-        store_state<NW, I8>(sa, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
-      if (fb >= 0 && fb == qb - gb * R)
-        store_state<NW, I8>(sb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
+    __syncthreads();
+    if (p.best_sem == 1) {  // every owner wave copies its part of the new best state
+      if (acta && L.src[ga] == qa - ga * R)
+        store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
+      if (actb && L.src[gb] == qb - gb * R)
+        store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
     }
-    __builtin_amdgcn_wave_barrier();
+    __syncthreads();
   }
 
   // ---- outputs
-  if (acta) store_state<NW, I8>(sa, nblk, n, p.ptype, p.spins + base_a);
-  if (actb) store_state<NW, I8>(sb, nblk, n, p.ptype, p.spins + base_b);
-  if (act) {
+  if (acta) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.spins + base_a);
+  if (actb) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.spins + base_b);
+  if (act && p.accepted) atomicAdd(p.accepted + read, acc);
+  if (act && a == 0) {
     p.energies[(size_t)read * R + rr] = E;
-    p.beta_idx[(size_t)read * R + rr] = s_bi[q];
-    if (p.accepted) atomicAdd(p.accepted + read, acc);
+    p.beta_idx[(size_t)read * R + rr] = L.bi[q];
     if (rr == 0 && p.best_sem == 1) {
       p.best_energy[read] = gE;
       p.best_sweeps[read] = gsw;
+      L.src[g] = gimp;
     }
   }
   if (p.best_sem == 1) {
     // never improved: best_state is still the view of states[0] (simulation.py:204)
-    __builtin_amdgcn_wave_barrier();
-    if (act && rr == 0) s_flag[g] = gimp;
-    __builtin_amdgcn_s_waitcnt(0xc07f);
-    __builtin_amdgcn_wave_barrier();
-    if (acta && qa - ga * R == 0 && !s_flag[ga])
-      store_state<NW, I8>(sa, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
-    if (actb && qb - gb * R == 0 && !s_flag[gb])
-      store_state<NW, I8>(sb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
+    __syncthreads();
+    if (acta && qa - ga * R == 0 && !L.src[ga])
+      store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
+    if (actb && qb - gb * R == 0 && !L.src[gb])
+      store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
   }
 }
 
@@ -513,18 +596,27 @@ __global__ void best_argmin_kernel(BParams p) {
   for (int i = 0; i < p.n; ++i) p.best_state[(size_t)read * p.n + i] = src[i];
 }
 
-template <int NP, bool I8>
-hipError_t launch_np(const BParams& p, int nw, int groups, hipStream_t st) {
-  const size_t lds = (size_t)kLdsPerWave * kWavesPerGroup;
-  dim3 grid(groups), block(64 * kWavesPerGroup);
+template <int NP, bool I8, int KW>
+hipError_t launch_kw(const BParams& p, int nw, int groups, hipStream_t st) {
+  dim3 grid(groups), block(64 * KW);
   switch (nw) {
-    case 4: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 4>), grid, block, lds, st, p); break;
-    case 8: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 8>), grid, block, lds, st, p); break;
-    case 16: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 16>), grid, block, lds, st, p); break;
-    case 32: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 32>), grid, block, lds, st, p); break;
+    case 4: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 4, KW>), grid, block, 0, st, p); break;
+    case 8: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 8, KW>), grid, block, 0, st, p); break;
+    case 16: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 16, KW>), grid, block, 0, st, p); break;
+    case 32: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 32, KW>), grid, block, 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int NP, bool I8>
+hipError_t launch_np(const BParams& p, int nw, int kw, int groups, hipStream_t st) {
+  switch (kw) {
+    case 1: return launch_kw<NP, I8, 1>(p, nw, groups, st);
+    case 2: return launch_kw<NP, I8, 2>(p, nw, groups, st);
+    case 4: return launch_kw<NP, I8, 4>(p, nw, groups, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
@@ -572,8 +664,14 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     return PYSA_AMD_ENOTSUP;
   }
   const int nblk = (n + kB - 1) / kB;
+  // Waves per replica group: enough groups x waves to give every SIMD of the
+  // device work (1024 on MI355X), at most 4, at least one block per wave.
+  const int groups = (reads + (64 / R) - 1) / (64 / R);
+  int kw = 1;
+  while (kw < 4 && 2 * kw <= nblk && groups * kw < 1024) kw *= 2;
+  const int kb = (nblk + kw - 1) / kw;
   int nw = 4;
-  while (2 * nw < nblk) nw *= 2;  // words of 2 blocks each: nblk <= 2 * nw
+  while (2 * nw < kb) nw *= 2;  // words of 2 blocks each: kb <= 2 * nw
   // Coupling encoding: exact where possible; `pieces` forces 2 or 3 bf16 pieces.
   if (b.exact_kind < 0) {
     std::vector<float> hJ((size_t)n * n);
@@ -639,7 +737,7 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   p.use_pt = use_pt;
   p.logw = logw;
   p.best_sem = best_sem;
-  p.refresh = 0;
+  p.kb = kb;
   p.k0 = (uint32_t)seed;
   p.k1 = (uint32_t)(seed >> 32);
   p.sweep_offset = sweep_offset;
@@ -668,8 +766,6 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     *why = "clearing counters";
     return PYSA_AMD_EHIP;
   }
-  const int waves = (reads + p.rpw - 1) / p.rpw;
-  const int groups = (waves + kWavesPerGroup - 1) / kWavesPerGroup;
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     *why = "creating events";
@@ -678,13 +774,13 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   (void)hipEventRecord(e0, st);
   hipError_t le;
   if (i8)
-    le = launch_np<1, true>(p, nw, groups, st);
+    le = launch_np<1, true>(p, nw, kw, groups, st);
   else if (np == 1)
-    le = launch_np<1, false>(p, nw, groups, st);
+    le = launch_np<1, false>(p, nw, kw, groups, st);
   else if (np == 2)
-    le = launch_np<2, false>(p, nw, groups, st);
+    le = launch_np<2, false>(p, nw, kw, groups, st);
   else
-    le = launch_np<3, false>(p, nw, groups, st);
+    le = launch_np<3, false>(p, nw, kw, groups, st);
   if (le == hipSuccess && best_sem == 0) {
     hipLaunchKernelGGL(best_argmin_kernel, dim3((reads + 63) / 64), dim3(64), 0, st, p);
     le = hipGetLastError();
