@@ -40,6 +40,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pysa_amd.h"
@@ -155,16 +156,20 @@ __device__ __forceinline__ void spins_i8(uint32_t x, int ptype, uint32_t (&d)[4]
   }
 }
 
-// Certified acceptance test: returns det_log(u) < x64 for u = (w + 0.5) 2^-32,
-// deciding in single precision when the gap exceeds the error bound of the
-// fast logarithm (and of the rounding of x to f32), else in exact f64.
-__device__ __forceinline__ bool log_less(uint32_t w, float x32, double x64) {
+// Certified acceptance test: returns det_log(u) < beta*dn (exact product in
+// f64) for u = (w + 0.5) 2^-32.  Decided in single precision when the gap
+// exceeds the error bound of the hardware logarithm plus the rounding of the
+// product to f32; otherwise (rarely) in exact f64, under a branch the wave
+// skips when no lane needs it.
+__device__ __forceinline__ bool log_less(uint32_t w, float beta, float dn32) {
+  const float x32 = beta * dn32;
   const float uf = ((float)w + 0.5f) * 2.3283064365386963e-10f;
   const float L = __builtin_amdgcn_logf(uf) * 0.69314718055994531f;
   const float m = 3.9e-6f * fabsf(L) + 9.6e-7f + 2.4e-7f * fabsf(x32);
-  if (L < x32 - m) return true;
-  if (L > x32 + m) return false;
-  return log_u32(w) < x64;
+  bool r = L < x32 - m;
+  const bool sure = r || (L > x32 + m);
+  if (__builtin_expect(!sure, 0)) r = log_u32(w) < (double)beta * (double)dn32;
+  return r;
 }
 
 // Spin bits of one MFMA column tile as seen by one lane: word c/2 holds the
@@ -325,6 +330,44 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
   unsigned long long acc = 0;
   __syncthreads();
 
+  // Prefetch pipeline.  Coupling fragments of the wave's K range stream
+  // through a ring of PF register stages; the first PF stages of the next
+  // block, and its J[blk][blk] / h[blk] share, are requested right after the
+  // current block's products so their latency hides behind the proposals.
+  constexpr int PF = 4;
+  constexpr int NST = I8 ? 1 : 2;  // fragments per K step of one block column
+  constexpr int NJB = (kB * kB + 64 * KW - 1) / (64 * KW);
+  typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
+  Frag ring[PF][NST][NP];
+  float jb_reg[NJB], h_reg = 0.f;
+  const int jn = max(0, min(kb, nblk - c0));
+  auto frag_ptr = [&](int c, int j, int st, int pc) -> const Frag* {
+    if (I8) return reinterpret_cast<const Frag*>(p.frag) + ((size_t)c * nblk + c0 + j) * 64 + lane;
+    return reinterpret_cast<const Frag*>(p.frag) +
+           (((size_t)c * (2 * nblk) + 2 * (c0 + j) + st) * NP + pc) * 64 + lane;
+  };
+  auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
+#pragma unroll
+    for (int st = 0; st < NST; ++st)
+#pragma unroll
+      for (int pc = 0; pc < NP; ++pc) dst[st][pc] = *frag_ptr(c, j, st, pc);
+  };
+  auto prefetch_block = [&](int c) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+      if (u < jn) load_stage(c, u, ring[u]);
+    const int pos0 = 32 * c;
+#pragma unroll
+    for (int e2 = 0; e2 < NJB; ++e2) {
+      const int e = tid + e2 * 64 * KW;
+      const int i = e >> 5, k = e & 31;
+      jb_reg[e2] = (e < kB * kB && pos0 + i < n && pos0 + k < n)
+                       ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
+    }
+    h_reg = (tid < kB && pos0 + tid < n) ? p.h[pos0 + tid] : 0.f;
+  };
+  prefetch_block(0);
+
   for (int s = 0; s < p.S; ++s) {
     const uint64_t sw = (uint64_t)s + p.sweep_offset;
     const float beta = act ? L.rb[q] : 0.f;
@@ -332,12 +375,13 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
     double dsum = 0.0;
     for (int c = 0; c < nblk; ++c) {
       const int buf = c & 1, oc = c / kb, pos0 = 32 * c;
-      // ---- phase 1a: stage J[blk][blk] (as J^T rows) and h[blk]
-      for (int e = tid; e < kB * kB; e += 64 * KW) {
-        const int i = e >> 5, k = e & 31;
-        L.jb[i * kB + k] = (pos0 + i < n && pos0 + k < n) ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
+      // ---- phase 1a: publish the prefetched J[blk][blk] (as J^T rows) and h[blk]
+#pragma unroll
+      for (int e2 = 0; e2 < NJB; ++e2) {
+        const int e = tid + e2 * 64 * KW;
+        if (e < kB * kB) L.jb[e] = jb_reg[e2];
       }
-      if (tid < kB) L.hb[tid] = (pos0 + tid < n) ? p.h[pos0 + tid] : 0.f;
+      if (tid < kB) L.hb[tid] = h_reg;
       // ---- phase 1b: the owner publishes the block's spins of all 64 replicas
       if (w == oc) {
         const int j = c - c0;
@@ -349,64 +393,50 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
       // ---- phase 1c: partial block fields over this wave's K range
       f32x16 acc0 = {}, acc1 = {};
       i32x16 iacc0 = {}, iacc1 = {};
-      const int jn = min(kb, nblk - c0);
-      if (I8) {
-        const i32x4* base = reinterpret_cast<const i32x4*>(p.frag) + ((size_t)c * nblk + c0) * 64 + lane;
-        i32x4 a_nx = (jn > 0) ? base[0] : i32x4{0, 0, 0, 0};
-        for (int j = 0; j < jn; ++j) {
-          const i32x4 a = a_nx;
-          if (j + 1 < jn) a_nx = base[(size_t)(j + 1) * 64];
-          uint32_t da[4], db[4];
-          spins_i8(get_half<NW>(sa, j), p.ptype, da);
-          spins_i8(get_half<NW>(sb, j), p.ptype, db);
-          i32x4 ba, bb;
+      for (int jb0 = 0; jb0 < jn; jb0 += PF) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            ba[e] = (int)da[e];
-            bb[e] = (int)db[e];
-          }
-          iacc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, ba, iacc0, 0, 0, 0);
-          iacc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, bb, iacc1, 0, 0, 0);
-        }
-      } else {
-        const bf16x8* base =
-            reinterpret_cast<const bf16x8*>(p.frag) + ((size_t)c * (2 * nblk) + 2 * c0) * NP * 64 + lane;
-        bf16x8 a_nx[2][NP];
-        if (jn > 0) {
+        for (int u = 0; u < PF; ++u) {
+          const int j = jb0 + u;
+          if (j >= jn) break;
+          Frag fa[NST][NP];
 #pragma unroll
-          for (int st = 0; st < 2; ++st)
+          for (int st = 0; st < NST; ++st)
 #pragma unroll
-            for (int pc = 0; pc < NP; ++pc) a_nx[st][pc] = base[(st * NP + pc) * 64];
-        }
-        for (int j = 0; j < jn; ++j) {
-          bf16x8 a[2][NP];
-#pragma unroll
-          for (int st = 0; st < 2; ++st)
-#pragma unroll
-            for (int pc = 0; pc < NP; ++pc) a[st][pc] = a_nx[st][pc];
-          if (j + 1 < jn) {
-#pragma unroll
-            for (int st = 0; st < 2; ++st)
-#pragma unroll
-              for (int pc = 0; pc < NP; ++pc)
-                a_nx[st][pc] = base[((size_t)(2 * (j + 1) + st) * NP + pc) * 64];
-          }
+            for (int pc = 0; pc < NP; ++pc) fa[st][pc] = ring[u][st][pc];
+          if (j + PF < jn) load_stage(c, j + PF, ring[u]);
           const uint32_t ha = get_half<NW>(sa, j), hb = get_half<NW>(sb, j);
-#pragma unroll
-          for (int st = 0; st < 2; ++st) {
+          if (I8) {
             uint32_t da[4], db[4];
-            spins_bf16((ha >> (8 * st)) & 0xFFu, p.ptype, da);
-            spins_bf16((hb >> (8 * st)) & 0xFFu, p.ptype, db);
-            u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
-            const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bb = __builtin_bit_cast(bf16x8, ub);
+            spins_i8(ha, p.ptype, da);
+            spins_i8(hb, p.ptype, db);
+            i32x4 ba, bb;
 #pragma unroll
-            for (int pc = 0; pc < NP; ++pc) {
-              acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[st][pc], ba, acc0, 0, 0, 0);
-              acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[st][pc], bb, acc1, 0, 0, 0);
+            for (int e = 0; e < 4; ++e) {
+              ba[e] = (int)da[e];
+              bb[e] = (int)db[e];
+            }
+            const i32x4 a8 = __builtin_bit_cast(i32x4, fa[0][0]);
+            iacc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, ba, iacc0, 0, 0, 0);
+            iacc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, bb, iacc1, 0, 0, 0);
+          } else {
+#pragma unroll
+            for (int st = 0; st < NST; ++st) {
+              uint32_t da[4], db[4];
+              spins_bf16((ha >> (8 * st)) & 0xFFu, p.ptype, da);
+              spins_bf16((hb >> (8 * st)) & 0xFFu, p.ptype, db);
+              u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
+              const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bb = __builtin_bit_cast(bf16x8, ub);
+#pragma unroll
+              for (int pc = 0; pc < NP; ++pc) {
+                const bf16x8 a16 = __builtin_bit_cast(bf16x8, fa[st][pc]);
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16, ba, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16, bb, acc1, 0, 0, 0);
+              }
             }
           }
         }
       }
+      prefetch_block(c + 1 < nblk ? c + 1 : 0);
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int i = 8 * (v >> 2) + 4 * h + (v & 3);
@@ -442,28 +472,31 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
         }
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
-          if (pos0 + k >= n) break;
-          const int ak = k / RPL, kl = k % RPL;
-          const bool up = (mb >> kl) & 1u;
-          const float fh = f[kl] + hv[kl];
-          // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
-          const float dn32 = (p.ptype == 0) ? (up ? 2.f : -2.f) * fh : (up ? fh : -fh);
-          bool take = dn32 >= 0.f;
-          if (!take && a == ak) take = log_less(uw[kl], beta * dn32, bd * (double)dn32);
-          take = take && act && (a == ak);
-          mb ^= (uint32_t)take << kl;
-          dsum += take ? -(double)dn32 : 0.0;
-          acc += take ? 1u : 0u;
-          const float dS_own = take ? ((p.ptype == 0) ? (up ? -2.f : 2.f) : (up ? -1.f : 1.f)) : 0.f;
-          const float dS = group_bcast<LPR>(dS_own, ak);
-          const float4* jr = reinterpret_cast<const float4*>(&L.jb[k * kB + r0]);
+          if (pos0 + k < n) {
+            const int ak = k / RPL, kl = k % RPL;
+            const bool up = (mb >> kl) & 1u;
+            const float fh = f[kl] + hv[kl];
+            // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
+            const float dn32 = (p.ptype == 0) ? (up ? 2.f : -2.f) * fh : (up ? fh : -fh);
+            // every lane evaluates the test; only the owner of row k keeps it
+            bool take = (dn32 >= 0.f) || log_less(uw[kl], beta, dn32);
+            take = take && act && (a == ak);
+            mb ^= (uint32_t)take << kl;
+            dsum += take ? -(double)dn32 : 0.0;
+            acc += take ? 1u : 0u;
+            const float dS_own =
+                take ? ((p.ptype == 0) ? (up ? -2.f : 2.f) : (up ? -1.f : 1.f)) : 0.f;
+            const float dS = group_bcast<LPR>(dS_own, ak);
+            const float4* jr = reinterpret_cast<const float4*>(&L.jb[k * kB + r0]);
 #pragma unroll
-          for (int m = 0; m < RPL / 4; ++m) {
-            const float4 j4 = jr[m];
-            f[4 * m] += j4.x * dS;
-            f[4 * m + 1] += j4.y * dS;
-            f[4 * m + 2] += j4.z * dS;
-            f[4 * m + 3] += j4.w * dS;
+            for (int m = 0; m < RPL / 4; ++m) {
+              const float4 j4 = jr[m];
+              // dS is 0, +-1 or +-2, so the fused form rounds like mul + add
+              f[4 * m] = __builtin_fmaf(j4.x, dS, f[4 * m]);
+              f[4 * m + 1] = __builtin_fmaf(j4.y, dS, f[4 * m + 1]);
+              f[4 * m + 2] = __builtin_fmaf(j4.z, dS, f[4 * m + 2]);
+              f[4 * m + 3] = __builtin_fmaf(j4.w, dS, f[4 * m + 3]);
+            }
           }
         }
         if (RPL == 32)
