@@ -116,10 +116,6 @@ __global__ void pack_i8_kernel(const float* __restrict__ J, int n, int nblk, int
 }
 
 // ------------------------------------------------------------ device util --
-__device__ __forceinline__ float rlf(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
 // 16 spin bits of one lane-half of a block: bf16 form keeps positions
 // {8h..8h+7} in the low byte and {16+8h..16+8h+7} in the high byte; int8
 // form keeps positions 16h..16h+15.
@@ -156,25 +152,22 @@ __device__ __forceinline__ void spins_i8(uint32_t x, int ptype, uint32_t (&d)[4]
   }
 }
 
-// Certified acceptance test: returns det_log(u) < beta*dn (exact product in
-// f64) for u = (w + 0.5) 2^-32.  Decided in single precision when the gap
-// exceeds the error bound of the hardware logarithm plus the rounding of the
-// product to f32; otherwise (rarely) in exact f64, under a branch the wave
-// skips when no lane needs it.
-__device__ __forceinline__ bool log_less(uint32_t w, float beta, float dn32) {
+// Certified acceptance test for dn < 0: returns det_log(u) < beta*dn (exact
+// product in f64) for u = (w + 0.5) 2^-32.  Decided in single precision when
+// the gap exceeds the error bound of the hardware logarithm plus the rounding
+// of the product to f32; otherwise (rarely) in exact f64, under a branch the
+// wave skips when no lane needs it.  `need` masks lanes whose answer is used.
+__device__ __forceinline__ bool log_less(uint32_t w, float beta, float dn32, bool need) {
   const float x32 = beta * dn32;
   const float uf = ((float)w + 0.5f) * 2.3283064365386963e-10f;
   const float L = __builtin_amdgcn_logf(uf) * 0.69314718055994531f;
   const float m = 3.9e-6f * fabsf(L) + 9.6e-7f + 2.4e-7f * fabsf(x32);
   bool r = L < x32 - m;
   const bool sure = r || (L > x32 + m);
-  if (__builtin_expect(!sure, 0)) r = log_u32(w) < (double)beta * (double)dn32;
+  if (__builtin_expect(need && !sure, 0)) r = log_u32(w) < (double)beta * (double)dn32;
   return r;
 }
 
-// Spin bits of one MFMA column tile as seen by one lane: word c/2 holds the
-// 16-bit half of blocks 2(c/2) and 2(c/2)+1.  A vector type keeps the words
-// in VGPRs under wave-uniform dynamic indexing.
 template <int NW>
 struct Bits {
   typedef uint32_t V __attribute__((ext_vector_type(NW)));
@@ -371,8 +364,9 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
   for (int s = 0; s < p.S; ++s) {
     const uint64_t sw = (uint64_t)s + p.sweep_offset;
     const float beta = act ? L.rb[q] : 0.f;
-    const double bd = (double)beta;
+    const float cu = (p.ptype == 0) ? 2.f : 1.f;  // |dn / (F + h)|
     double dsum = 0.0;
+    uint32_t nacc = 0;
     for (int c = 0; c < nblk; ++c) {
       const int buf = c & 1, oc = c / kb, pos0 = 32 * c;
       // ---- phase 1a: publish the prefetched J[blk][blk] (as J^T rows) and h[blk]
@@ -393,11 +387,13 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
       // ---- phase 1c: partial block fields over this wave's K range
       f32x16 acc0 = {}, acc1 = {};
       i32x16 iacc0 = {}, iacc1 = {};
-      for (int jb0 = 0; jb0 < jn; jb0 += PF) {
+      constexpr int kUnrollJ = NW <= 8 ? (2 * NW + PF - 1) / PF : 1;
+#pragma unroll kUnrollJ
+      for (int jb0 = 0; jb0 < 2 * NW; jb0 += PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int j = jb0 + u;
-          if (j >= jn) break;
+          if (j >= jn) continue;
           Frag fa[NST][NP];
 #pragma unroll
           for (int st = 0; st < NST; ++st)
@@ -475,18 +471,17 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
           if (pos0 + k < n) {
             const int ak = k / RPL, kl = k % RPL;
             const bool up = (mb >> kl) & 1u;
-            const float fh = f[kl] + hv[kl];
             // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
-            const float dn32 = (p.ptype == 0) ? (up ? 2.f : -2.f) * fh : (up ? fh : -fh);
+            const float sg = up ? cu : -cu;
+            const float dn32 = (f[kl] + hv[kl]) * sg;
+            const bool own = act && (a == ak);
             // every lane evaluates the test; only the owner of row k keeps it
-            bool take = (dn32 >= 0.f) || log_less(uw[kl], beta, dn32);
-            take = take && act && (a == ak);
+            const bool lt = log_less(uw[kl], beta, dn32, own & (dn32 < 0.f));
+            const bool take = own & ((dn32 >= 0.f) | lt);
             mb ^= (uint32_t)take << kl;
             dsum += take ? -(double)dn32 : 0.0;
-            acc += take ? 1u : 0u;
-            const float dS_own =
-                take ? ((p.ptype == 0) ? (up ? -2.f : 2.f) : (up ? -1.f : 1.f)) : 0.f;
-            const float dS = group_bcast<LPR>(dS_own, ak);
+            nacc += take ? 1u : 0u;
+            const float dS = group_bcast<LPR>(take ? -sg : 0.f, ak);
             const float4* jr = reinterpret_cast<const float4*>(&L.jb[k * kB + r0]);
 #pragma unroll
             for (int m = 0; m < RPL / 4; ++m) {
@@ -515,6 +510,7 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
       }
     }
     // ---- end of sweep: energy (simulation.py:157-158) and best states
+    acc += nacc;
 #pragma unroll
     for (int o = 1; o < LPR; o <<= 1) dsum += __shfl_xor(dsum, o, 64);
     E = (float)((double)E + dsum);
