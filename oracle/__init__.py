@@ -1,30 +1,32 @@
 """ORACLE -- test infrastructure only.
 
 CPU restatement of PySA's Metropolis / parallel-tempering path and AIS
-(oracle/oracle.cpp, built by oracle/Makefile into oracle/_build/liboracle.so)
-plus a pure-Python restatement for tiny replay cases (oracle/loops.py).  Only
-tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use this
-package, and only as the checker / CPU baseline.
+(oracle/oracle.cpp, built by oracle/Makefile into oracle/_build/liboracle.so).
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker / CPU baseline.
 
 What pins each oracle function (tests named in brackets):
   philox           Random123 known-answer vectors      [test_oracle_rng]
-  det_log          |det_log(x) - log(x)| <= 2 ulp over (0,1) and wider,
-                   exact at powers of two               [test_oracle_rng]
+  det_log          |det_log(x) - log(x)| <= 2 ulp over the engine's uniforms
+                   and (1e-300, 1e300), powers of two   [test_oracle_rng]
   perm (Feistel)   bijection of [0, n) for many n       [test_oracle_rng]
-  energy           brute force / closed forms (planted Wishart, all-ones)
-                                                        [test_oracle_semantics]
-  update_spin/sweep  beta=0 forced flips + E+dE consistency (the reference's
-                   own invariants, tests_ising_sequential.py:100-179),
-                   hand-worked 3-spin known answers, golden trajectories
-                   written by the reference under tests/golden/
-                                                        [test_oracle_semantics,
-                                                         test_oracle_golden]
-  pt               multiset invariance (utils tests), detailed balance of slot
-                   occupancy vs exact Boltzmann weights, golden trajectories
-  simulation best  brute-force lower bound (tests_*:227-290), golden vectors
-                   incl. the sequential aliasing quirk
-  get_log_omega    closed form on hand-made ladders, golden vectors
-  ais              exact log Z by enumeration (statistical), golden vectors
+  energy           closed form on random problems, brute-force ground state
+                   of the planted Wishart generator     [test_oracle_semantics]
+  update_spin/sweep  hand-worked 3-spin Ising and QUBO sweeps (acceptance at
+                   dn = 0, strict log_r < beta*dn), beta = 0 forced flips and
+                   tracked = recomputed energy (the reference's own
+                   invariants, tests_ising_sequential.py:100-179)
+  pt               hand-worked two-replica exchange, hot-end-first chained
+                   order (closed-form rotation when every swap is accepted),
+                   permutation invariance, per-slot chi-square against exact
+                   Boltzmann weights
+  simulation best  hand-worked best/best_sweeps, the sequential aliasing quirk
+                   (pysa/simulation.py:204-206) as a known answer, prefix
+                   replay of best_sweeps, brute-force lower bound
+                   (tests_*:227-290), the reference's N=50 Wishart fixture and
+                   its ground-state energy
+  get_log_omega    hand-worked two-sweep values
+  ais              log Z against exact enumeration (both weight orderings)
 """
 
 import ctypes

@@ -1,0 +1,324 @@
+"""Pins of the oracle's Metropolis / PT / log-omega / AIS semantics.
+
+Each check is independent of the oracle's own code: hand-worked known
+answers, the invariants the reference's tests assert
+(tests/tests_ising_sequential.py and tests_qubo_sequential.py in PySA),
+brute force over all states, exact Boltzmann statistics and exact partition
+functions, and the reference's own data fixture (the N=50 planted Wishart
+instance of its tutorials, with its ground-state energy).
+"""
+
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+from scipy.special import logsumexp
+
+import oracle
+from pysa_amd import instances
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def all_states(n, ptype):
+    b = np.array(list(itertools.product([0, 1], repeat=n)), dtype=np.int8)
+    return b if ptype == 1 else (2 * b - 1).astype(np.int8)
+
+
+def energies_np(J, h, s):
+    s = s.astype(np.float64)
+    return 0.5 * np.einsum("ki,ij,kj->k", s, J.astype(np.float64), s) + s @ h.astype(np.float64)
+
+
+def run(J, h, ptype, betas, st, sweeps, **kw):
+    J = np.asarray(J, dtype=np.float32)
+    h = np.asarray(h, dtype=np.float32)
+    st = np.asarray(st, dtype=np.int8)
+    en = energies_np(J, h, st.reshape(-1, st.shape[-1])).reshape(st.shape[:2]).astype(np.float32)
+    return oracle.metropolis(J, h, ptype, np.asarray(betas, np.float32), st, en, sweeps, **kw)
+
+
+# A 3-spin problem worked by hand below.
+J3 = np.array([[0, 1, -2], [1, 0, 3], [-2, 3, 0]], dtype=np.float32)
+H3 = np.array([1, 0, -1], dtype=np.float32)
+
+
+def test_energy_closed_form():
+    rng = np.random.default_rng(0)
+    J = rng.normal(size=(40, 40))
+    J = (J + J.T) / 2
+    np.fill_diagonal(J, 0)
+    h = rng.normal(size=40)
+    for ptype in (0, 1):
+        s = all_states(10, ptype)[::7]
+        s = np.concatenate([s, s[:, ::-1], s, s[:, ::-1]], axis=1)
+        assert np.allclose(oracle.energy_f64(J, h, s), energies_np(J, h, s), rtol=1e-13, atol=1e-12)
+    # E(s) for s = (1, 1, 1) of the 3-spin problem: J s = (-1, 4, 1)
+    assert oracle.energy_f64(J3, H3, np.ones((1, 3), np.int8))[0] == 2.0
+
+
+def test_kat_three_spin_ising_sweep():
+    """One sequential sweep at beta = 1 from s = (1, 1, 1), E = 2.
+    pos 0: dn = 2*1*(J0.s + h0) = 2*(-1 + 1) = 0  -> accepted (dn >= 0)
+    pos 1: s = (-1, 1, 1), J1.s = 2, dn = 4       -> accepted, dE = -4
+    pos 2: s = (-1,-1, 1), J2.s = -1, dn = 2*(-2) = -4: accepted iff log_r < -4
+    """
+    for lr2, want, e_want, acc in ((-3.0, [-1, -1, 1], -2.0, 2), (-4.0, [-1, -1, 1], -2.0, 2),
+                                   (-4.5, [-1, -1, -1], 2.0, 3)):
+        rep = dict(log_r=np.array([-10.0, -10.0, lr2]).reshape(1, 1, 1, 3))
+        out = run(J3, H3, 0, [1.0], np.ones((1, 1, 3)), 1, use_pt=False, replay=rep)
+        assert out["states"][0, 0].tolist() == want
+        assert out["energies"][0, 0] == e_want
+        assert out["accepted"][0] == acc
+
+
+def test_kat_three_spin_qubo_sweep():
+    """x = (1, 0, 1), E = -2, beta = 0.5.
+    pos 0: J0.x + h0 = -2 + 1 = -1, dn = (2*1-1)*(-1) = -1: needs log_r < -0.5
+    pos 1: J1.x + h1 = 4, dn = (2*0-1)*4 = -4: needs log_r < -2
+    pos 2: (after pos 1 flips) J2.x + h2 = 1 - 1 = 0, dn = 0 -> accepted
+    """
+    rep = dict(log_r=np.array([-0.4, -3.0, -9.0]).reshape(1, 1, 1, 3))
+    out = run(J3, H3, 1, [0.5], np.array([[[1, 0, 1]]]), 1, use_pt=False, replay=rep)
+    assert out["states"][0, 0].tolist() == [1, 1, 0]
+    assert out["energies"][0, 0] == 2.0
+    assert out["accepted"][0] == 2
+
+
+def test_kat_pt_log_omega_and_best():
+    """Two replicas, betas (1, 0), two sweeps; every number worked by hand.
+
+    Sweep 0: replica 0 (beta 1) log_r = (-10, -10, -3) -> (-1,-1, 1), E = -2;
+             replica 1 (beta 0) flips every spin      -> (-1,-1,-1), E =  2.
+             PT pair (1, 0): de = (2 - (-2)) * (0 - 1) = -4; log u = -5 < -4:
+             swap, beta_idx = (1, 0).  log_omega = -(1 - 0) * E[beta_idx[0]] = -2.
+    Sweep 1: replica 0 now runs at beta 0: flips all -> (1, 1,-1), E = 2;
+             replica 1 at beta 1 with log_r = 0 (only dn >= 0 accepted):
+             pos 0 dn = -4 rejected, pos 1 dn = 8 accepted, pos 2 dn = -8
+             rejected -> (-1, 1,-1), E = -6.
+             PT: de = (2 - (-6)) * (0 - 1) = -8; log u = -1: no swap.
+             log_omega = -E[beta_idx[0]] = -E[1] = 6.
+    Best: -6 at sweep 1 (replica 1) under both best semantics.
+    """
+    lr = np.zeros((1, 2, 2, 3))
+    lr[0, 0, 0] = [-10, -10, -3]
+    lr[0, 0, 1] = [-1, -1, -1]
+    lr[0, 1, 0] = [-1, -1, -1]
+    lr[0, 1, 1] = [0, 0, 0]
+    rep = dict(log_r=lr, pt_logu=np.array([[[-5.0], [-1.0]]]))
+    for sem in (0, 1):
+        out = run(J3, H3, 0, [1.0, 0.0], np.ones((1, 2, 3)), 2, replay=rep, best_sem=sem,
+                  log_omega=True)
+        assert out["states"][0].tolist() == [[1, 1, -1], [-1, 1, -1]]
+        assert out["energies"][0].tolist() == [2.0, -6.0]
+        assert out["beta_idx"][0].tolist() == [1, 0]
+        assert out["log_omegas"][0].tolist() == [-2.0, 6.0]
+        assert out["best_state"][0].tolist() == [-1, 1, -1]
+        assert out["best_energy"][0] == -6.0
+        assert out["best_sweeps"][0] == 1
+
+
+def test_kat_sequential_best_aliases_replica_zero():
+    """simulation_sequential starts its best as a view of states[0]
+    (pysa/simulation.py:204-206): when no replica ever goes strictly below
+    replica 0's initial energy, best_state is replica 0's *final* state while
+    best_energy stays its initial energy.  simulation_parallel copies."""
+    E = energies_np(J3, H3, all_states(3, 0))
+    gs = all_states(3, 0)[np.argmin(E)]
+    st = np.stack([gs, np.ones(3, np.int8)])[None]
+    rep = dict(log_r=np.full((1, 1, 2, 3), -1.0))
+    for sem, want in ((0, gs.tolist()), (1, (-gs).tolist())):
+        out = run(J3, H3, 0, [0.0, 1.0], st, 1, replay=rep, best_sem=sem, use_pt=False)
+        assert out["states"][0, 0].tolist() == (-gs).tolist()  # beta 0 flips every spin
+        assert out["best_state"][0].tolist() == want
+        assert out["best_energy"][0] == E.min()
+        assert out["best_sweeps"][0] == 0
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_beta_zero_flips_every_spin(ptype, order):
+    """tests_*_sequential.py:100-179: at beta = 0 every proposal is accepted,
+    and the tracked energy stays equal to the recomputed one."""
+    rng = np.random.default_rng(3 + ptype)
+    n, R, S = 37, 4, 5
+    A = np.triu(rng.integers(-4, 5, (n, n)), 1)
+    J, h = A + A.T, rng.integers(-3, 4, n)
+    b = rng.integers(0, 2, (2, R, n))
+    st = b if ptype else 2 * b - 1
+    out = run(J, h, ptype, np.zeros(R), st, S, order=order, seed=17, use_pt=False)
+    flipped = (1 - st) if ptype else -st  # S is odd
+    assert np.array_equal(out["states"], flipped)
+    assert np.all(out["accepted"] == n * R * S)
+    e = energies_np(J, h, out["states"].reshape(-1, n)).reshape(2, R)
+    assert np.array_equal(out["energies"], e.astype(np.float32))
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+def test_tracked_energy_matches_recomputed(ptype):
+    rng = np.random.default_rng(10 + ptype)
+    n, R, S = 50, 8, 40
+    A = np.triu(rng.integers(-3, 4, (n, n)), 1)
+    J, h = A + A.T, rng.integers(-2, 3, n)
+    b = rng.integers(0, 2, (3, R, n))
+    st = b if ptype else 2 * b - 1
+    betas = np.geomspace(2.0, 0.05, R)
+    for order in (0, 1):
+        out = run(J, h, ptype, betas, st, S, order=order, seed=5)
+        e = energies_np(J, h, out["states"].reshape(-1, n)).reshape(3, R)
+        assert np.array_equal(out["energies"], e.astype(np.float32))
+        be = energies_np(J, h, out["best_state"])
+        assert np.array_equal(out["best_energy"], be.astype(np.float32))
+        for r in range(3):  # PT only permutes temperatures
+            assert sorted(out["beta_idx"][r]) == list(range(R))
+
+
+def test_pt_chain_order_all_swaps_accepted():
+    """With J = h = 0 every energy is 0, every exchange is accepted and the
+    chained pass from the hot end (pysa/utils.py:44-58) rotates beta_idx by
+    one slot per sweep: beta_idx[j] = (j - S) mod R."""
+    R = 5
+    for S in (1, 2, 3, 7):
+        out = run(np.zeros((4, 4)), np.zeros(4), 0, np.geomspace(3, 0.3, R),
+                  np.ones((1, R, 4)), S, seed=1)
+        assert out["beta_idx"][0].tolist() == [(j - S) % R for j in range(R)]
+
+
+def test_best_sweeps_prefix_consistency():
+    """best_sweeps is the 0-based sweep at which best_state was reached:
+    replaying only the first best_sweeps + 1 sweeps ends with some replica
+    holding exactly (best_state, best_energy)."""
+    rng = np.random.default_rng(21)
+    n, R, S, reads = 24, 6, 30, 6
+    A = np.triu(rng.integers(-2, 3, (n, n)), 1)
+    J, h = A + A.T, rng.integers(-1, 2, n)
+    st = 2 * rng.integers(0, 2, (reads, R, n)) - 1
+    lr = np.log(rng.random((reads, S, R, n)))
+    pt = np.log(rng.random((reads, S, R - 1)))
+    betas = np.geomspace(3.0, 0.3, R)
+    full = run(J, h, 0, betas, st, S, replay=dict(log_r=lr, pt_logu=pt))
+    for r in range(reads):
+        k = int(full["best_sweeps"][r]) + 1
+        pre = run(J, h, 0, betas, st[r:r + 1], k,
+                  replay=dict(log_r=lr[r:r + 1, :k], pt_logu=pt[r:r + 1, :k]))
+        hit = [np.array_equal(pre["states"][0, q], full["best_state"][r]) and
+               pre["energies"][0, q] == full["best_energy"][r] for q in range(R)]
+        assert any(hit), r
+
+
+def test_brute_force_lower_bound():
+    """tests_*_sequential.py:227-290: the best energy is never below the exact
+    minimum, and a modest PT run reaches it."""
+    rng = np.random.default_rng(8)
+    n = 14
+    for ptype in (0, 1):
+        A = np.triu(rng.integers(-5, 6, (n, n)), 1)
+        J, h = A + A.T, rng.integers(-3, 4, n)
+        emin = energies_np(J, h, all_states(n, ptype)).min()
+        b = rng.integers(0, 2, (4, 8, n))
+        out = run(J, h, ptype, np.geomspace(2.0, 0.1, 8), b if ptype else 2 * b - 1, 200,
+                  order=1, seed=2)
+        assert np.all(out["best_energy"] >= emin)
+        assert np.all(out["best_energy"] == emin)
+
+
+def test_planted_wishart_ground_state_bruteforce():
+    """The planted instance generator: all-ones is a ground state and its
+    energy is the closed form 1/2 sum_ij P_ij (brute force over 2^16)."""
+    P, e_gs = instances.planted_wishart(16, 0.5, 7)
+    J = P.copy()
+    np.fill_diagonal(J, 0)
+    h = np.diag(P).copy()
+    E = energies_np(J, h, all_states(16, 0))
+    assert np.isclose(E.min(), e_gs, rtol=1e-12, atol=1e-9)
+    assert np.isclose(energies_np(J, h, np.ones((1, 16)))[0], e_gs, rtol=1e-12)
+
+
+def test_tutorial_wishart_n50_ground_state():
+    """The reference's N=50 planted Wishart fixture (tutorials/, chook format
+    i j J_ij for i < j) with its listed ground-state energy -1534000
+    (sum_{i<j} J_ij s_i s_j).  PySA's convention with P = (W + W^T)/2 halves
+    it: -767000.  PT reaches it and never goes below it."""
+    d = np.loadtxt(os.path.join(GOLDEN, "wishart_planting_N_50_alpha_0.50_inst_1.txt"))
+    with open(os.path.join(GOLDEN, "wishart_N_50_gs_energies.txt")) as f:
+        gs_chook = float(f.read().split()[1])
+    n = 50
+    W = np.zeros((n, n))
+    W[d[:, 0].astype(int), d[:, 1].astype(int)] = d[:, 2]
+    P = ((W + W.T) / 2).astype(np.float32)
+    J = P.copy()
+    np.fill_diagonal(J, 0)
+    h = np.diag(P).copy()
+    R, reads = 24, 16
+    betas = (1 / np.geomspace(1000, 100000, R)).astype(np.float32)
+    rng = np.random.default_rng(0)
+    st = 2 * rng.integers(0, 2, (reads, R, n)) - 1
+    out = run(J, h, 0, betas, st, 4000, order=0, seed=3, nthreads=8)
+    e = energies_np(J, h, out["best_state"])
+    assert np.array_equal(e.astype(np.float32), out["best_energy"])
+    assert e.min() == gs_chook / 2
+    assert np.all(e >= gs_chook / 2)
+
+
+def test_detailed_balance_per_slot():
+    """With PT on, the replica held in slot k samples the Boltzmann
+    distribution at betas[k]: chi-square of its energy levels against exact
+    enumeration, at every slot."""
+    rng = np.random.default_rng(4)
+    n, R, reads, S = 6, 4, 3000, 40
+    A = np.triu(rng.integers(-2, 3, (n, n)), 1)
+    J, h = A + A.T, rng.integers(-1, 2, n)
+    betas = np.array([1.0, 0.7, 0.45, 0.25], dtype=np.float32)
+    st = 2 * rng.integers(0, 2, (reads, R, n)) - 1
+    out = run(J, h, 0, betas, st, S, order=1, seed=11, nthreads=8)
+    allE = energies_np(J, h, all_states(n, 0))
+    levels = np.unique(allE)
+    for k in range(R):
+        w = np.exp(-float(betas[k]) * (allE - allE.min()))
+        p = np.array([w[allE == L].sum() for L in levels]) / w.sum()
+        samp = out["energies"][np.arange(reads), out["beta_idx"][:, k]]
+        obs = np.array([(samp == L).sum() for L in levels])
+        keep = p * reads >= 5
+        exp_ = p[keep] * reads
+        o = obs[keep]
+        chi2 = ((o - exp_ * o.sum() / exp_.sum()) ** 2 / (exp_ * o.sum() / exp_.sum())).sum()
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (k, chi2)
+
+
+def rbm_problem(nv, nh, seed, ws=0.3, bs=0.2):
+    rng = np.random.default_rng(seed)
+    W = rng.normal(0, ws, (nv, nh))
+    b = rng.normal(0, bs, nv)
+    c = rng.normal(0, bs, nh)
+    n = nv + nh
+    P = np.zeros((n, n))
+    P[:nv, nv:] = W
+    P[nv:, :nv] = W.T
+    P[np.arange(nv), np.arange(nv)] = b
+    P[nv + np.arange(nh), nv + np.arange(nh)] = c
+    return P
+
+
+@pytest.mark.parametrize("ordering", [0, 1])
+def test_ais_log_z_matches_enumeration(ordering):
+    """Uniform-reference AIS: log Z = logmeanexp(log w) + N log 2 agrees with
+    exact enumeration of exp(-E) (pysa/sa.py:543-551 for the estimator)."""
+    P = rbm_problem(6, 5, 1)
+    n = P.shape[0]
+    J = P.copy()
+    np.fill_diagonal(J, 0)
+    h = np.diag(P).copy()
+    logz = logsumexp(-energies_np(J, h, all_states(n, 0)))
+    chains, T = 4000, 200
+    temps = np.array([(t + 1) / T for t in range(T)], dtype=np.float64)
+    rng = np.random.default_rng(2)
+    st0 = (2 * rng.integers(0, 2, (chains, n)) - 1).astype(np.int8)
+    _, lw = oracle.ais(J.astype(np.float64), h.astype(np.float64), 0, temps, st0, order=1,
+                       ordering=ordering, seed=9, nthreads=8)
+    est = logsumexp(lw) - math.log(chains) + n * math.log(2)
+    boots = [logsumexp(lw[rng.integers(0, chains, chains)]) for _ in range(200)]
+    se = np.std(boots)
+    assert abs(est - logz) < 5 * se + 0.02, (est, logz, se)
