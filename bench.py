@@ -104,6 +104,20 @@ class Workload:
         return e, gap
 
 
+def copy_calibration(nbytes):
+    """Device-to-device copies of a buffer of known size (the HIP runtime's
+    copy kernel), so that profiler byte counters can be scaled: run under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE beside the engine."""
+    from pysa_amd import _lib
+    a = _lib.DeviceArray(nbytes, np.uint8)
+    b = _lib.DeviceArray(nbytes, np.uint8)
+    for _ in range(3):
+        b.copy_from(a)
+    log(f"[bench] copied {nbytes} bytes device-to-device 3 times")
+    a.free()
+    b.free()
+
+
 def run_form(wl, form_name, steps, warmup, seed):
     L = wl.lib
     form = {"plain": L.FORM_PLAIN, "blocked": L.FORM_BLOCKED}[form_name]
@@ -137,7 +151,8 @@ def run_form(wl, form_name, steps, warmup, seed):
         "acceptance": acc / (wl.updates * steps),
         "p_success": p0,
         "tts99_s": tts(p0, t_read),
-        "gap_tts99_s": {f"{g:g}": tts(float(np.mean(gaps <= g)), t_read) for g in (0.01, 0.05)},
+        "gap_tts99_s": {f"{g:g}": tts(float(np.mean(gaps <= g)), t_read)
+                        for g in (0.01, 0.05, 0.1, 0.15, 0.2)},
         "best_gap_median": float(np.median(gaps)),
         "t_read_s": t_read,
     }
@@ -189,6 +204,10 @@ def main():
     ap.add_argument("--forms", default="auto", help="auto | plain | blocked | both")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1003)
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="timed steps of the plain form when it is the comparison, not the headline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run device-to-device copies of BYTES (scales profiler byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -197,14 +216,18 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+
     from pysa_amd import _lib
     forms = {"auto": ["plain", "blocked"] if _lib.blocked_available() else ["plain"],
              "both": ["plain", "blocked"], "plain": ["plain"], "blocked": ["blocked"]}[args.forms]
     wl = Workload(args.reads, args.replicas, args.sweeps, args.seed)
     res = {}
+    head = "blocked" if "blocked" in forms else "plain"
     for f in forms:
-        res[f] = run_form(wl, f, args.steps, args.warmup, args.seed)
-    head = "blocked" if "blocked" in res else "plain"
+        k = args.steps if f == head else min(args.steps, args.plain_steps)
+        res[f] = run_form(wl, f, k, args.warmup, args.seed)
     H = res[head]
     n = wl.n
     if head == "plain":

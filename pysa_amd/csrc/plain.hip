@@ -123,6 +123,21 @@ struct Chain {
   }
   __device__ bool up_at(int pos) const { return (rl(bits, lane_of(pos)) >> slot_of(pos)) & 1; }
 
+  // f[g] for a wave-uniform slot g, as a select tree on g's bits: register
+  // indices stay compile-time constants, so the fields never leave VGPRs.
+  __device__ __forceinline__ T pick(int g) const {
+    T v[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) v[k] = f[k];
+#pragma unroll
+    for (int half = G / 2; half >= 1; half >>= 1) {
+      const bool hi = (g & half) != 0;
+#pragma unroll
+      for (int k = 0; k < half; ++k) v[k] = hi ? v[k + half] : v[k];
+    }
+    return v[0];
+  }
+
   // F = J s accumulated in f64 and rounded once to T; returns
   // E = s.(F/2 + h) in f64 (the periodic recomputation that bounds drift).
   __device__ double refresh(const T* __restrict__ Jt, const T* h, int n, int ptype) {
@@ -280,7 +295,7 @@ __device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const 
       const double lr = rl(lr_l, t);
       const int pl = lane_of(pos), ps = slot_of(pos);
       const bool up = (rl(ch.bits, pl) >> ps) & 1;
-      const T fh = rl(ch.f[ps], pl) + hs[pos];
+      const T fh = rl(ch.pick(ps), pl) + hs[pos];
       double dn;
       if (decide<T, AIS>(c.ptype, up, fh, lr, (double)beta, c.ref, c.lratio, dn)) {
         sum += -dn;

@@ -1,0 +1,95 @@
+"""AIS on the GPU (pysa/simulation.py:266-317, pysa/sa.py:432-553): the
+kernel against the oracle, and log Z against exact enumeration."""
+
+import math
+
+import numpy as np
+import pytest
+from scipy.special import logsumexp
+
+import oracle
+from pysa_amd import instances
+from pysa_amd.sa import Solver
+
+from helpers import engine_ais, int_problem, random_spins
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("ref", [0, 1])
+@pytest.mark.parametrize("ordering", [0, 1])
+@pytest.mark.parametrize("order", [0, 1])
+def test_ais_replay_matches_oracle(ptype, ref, ordering, order):
+    rng = np.random.default_rng(50 + 8 * ptype + 4 * ref + 2 * ordering + order)
+    n, chains, nt = 37, 9, 25
+    J, h = int_problem(n, rng)
+    J, h = J.astype(np.float64), h.astype(np.float64)
+    temps = np.array([(t + 1) / nt for t in range(nt)])
+    st = random_spins((chains, n), rng, ptype)
+    rep = dict(log_r=np.log(rng.random((chains, nt, n))))
+    if order:
+        rep["perm"] = np.argsort(rng.random((chains, nt, n)), axis=-1).astype(np.int32)
+    want = oracle.ais(J, h, ptype, temps, st, ref=ref, p=0.3, order=order, ordering=ordering,
+                      replay=rep)
+    got = engine_ais(J, h, ptype, temps, st, ref=ref, p=0.3, order=order, ordering=ordering,
+                     replay=rep)
+    assert np.array_equal(got[0], want[0])
+    assert np.allclose(got[1], want[1], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_ais_philox_matches_oracle(dtype, order):
+    rng = np.random.default_rng(7)
+    n, chains, nt = 64, 40, 30
+    J, h = int_problem(n, rng)
+    J, h = J.astype(dtype), h.astype(dtype)
+    temps = np.array([(t + 1) / nt for t in range(nt)], dtype=dtype)
+    st = random_spins((chains, n), rng, 0)
+    want = oracle.ais(J, h, 0, temps, st, order=order, seed=123, nthreads=8)
+    got = engine_ais(J, h, 0, temps, st, order=order, seed=123)
+    assert np.array_equal(got[0], want[0])
+    assert np.allclose(got[1], want[1], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("ordering", ["reference", "neal"])
+def test_ais_log_z_exact_rbm_14_10(ordering):
+    """The exact-enumeration case: 14 visible + 10 hidden spins with the
+    benchmark's weight distributions; exact log Z sums the 2^14 visible states
+    with the hidden units summed out in closed form (that formula is checked
+    against full enumeration in test_host).  AIS with a uniform reference must
+    agree within its bootstrap error."""
+    nv, nh = 14, 10
+    P = instances.rbm_ising(nv, nh, seed=1005)
+    logz = instances.rbm_log_z(P, nv)
+    np.random.seed(3)
+    s = Solver(P.astype(np.float64), "ising", "float64")
+    chains = 4096
+    logzf, samples, omegas = s.annealed_importance(1000, chains, "uniform", "random", seed=77,
+                                                   ordering=ordering)
+    assert samples.shape == (chains, nv + nh)
+    assert np.isclose(np.sum(omegas), 1.0)
+    lw = np.log(np.asarray(omegas))
+    rng = np.random.default_rng(0)
+    boots = [logsumexp(lw[rng.integers(0, chains, chains)]) - math.log(chains)
+             for _ in range(200)]
+    se = float(np.std(boots))
+    assert abs(logzf - logz) < 5 * se + 5e-3, (logzf, logz, se)
+
+
+def test_ais_log_z_bernoulli_reference():
+    """Bernoulli reference (pysa/ais.py:175-226, log Z0 = 0) on a small RBM at
+    beta = 0.8 (couplings scaled as pysa/sa.py:511-512)."""
+    nv, nh = 8, 6
+    P = instances.rbm_ising(nv, nh, wstd=0.3, bstd=0.2, seed=9).astype(np.float64)
+    logz = instances.rbm_log_z(P, nv, beta=0.8)
+    np.random.seed(4)
+    s = Solver(P, "ising", "float64")
+    logzf, _, omegas = s.annealed_importance(400, 4096, "bernoulli", "sequential",
+                                             reference_args=[0.4], beta=0.8, seed=5,
+                                             ordering="neal")
+    lw = np.log(np.asarray(omegas))
+    rng = np.random.default_rng(1)
+    se = float(np.std([logsumexp(lw[rng.integers(0, 4096, 4096)]) for _ in range(200)]))
+    assert abs(logzf - logz) < 5 * se + 5e-3, (logzf, logz, se)
