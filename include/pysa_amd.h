@@ -156,6 +156,41 @@ int pysa_amd_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
 /* E = s.(J s / 2 + h) in f64 for `count` states (pysa/ising.py:32-37). */
 int pysa_amd_energy(int64_t handle, const int8_t* states, int64_t count, double* out);
 
+/*
+ * Many independent instances of different sizes in one launch.  PySA has no
+ * batched entry: a batch of problems is a caller's Python loop over Solver
+ * objects, each running pysa/sa.py:92-430 with its per-read loop
+ * (pysa/sa.py:425-427) over pysa/simulation.py:123-236.  This replaces that
+ * whole loop for float32 problems whose couplings are an integer matrix times
+ * a per-instance scale (e.g. Sherrington-Kirkpatrick +-1/sqrt(n)).
+ *
+ * Instance k (0 <= k < count), all arrays in device memory:
+ *   n[k]        spins, 1..2048
+ *   signs + j_off[k]  int8 S_k, n[k] rows of ld[k] bytes (ld a multiple of 16,
+ *               j_off 16-aligned, padding zero); J_k = scale[k] * S_k must be
+ *               symmetric with a zero diagonal
+ *   h + h_off[k]  n[k] float32 local fields, or h == NULL for h = 0
+ *   betas + k*R   its ladder (slot order, float32)
+ *   s_off[k]    sum_{j<k} reads*R*n[j]: offset of its states
+ *               (reads*R*n[k] int8 at states + s_off[k]) and, divided by R,
+ *               of its best states (reads*n[k] int8 at best_state + s_off[k]/R)
+ * Per (instance, read) g = k*reads + read: energies/beta_idx [g*R ..],
+ * best_energy/best_sweeps/accepted [g].  Local fields are exact integers
+ * S_k s; the field of spin i is the float32 value scale*c_i + h_i.  Randomness
+ * is the Philox stream of pysa_amd_metropolis with read index g.
+ *   R <= 16 (R <= 8 when some n > 1024), order 0-2, best_semantics as in
+ *   pysa_amd_metropolis.
+ */
+int pysa_amd_batched_device(int32_t count, const int32_t* n, const int32_t* ld,
+                            const int64_t* j_off, const int8_t* signs, const float* scale,
+                            const float* h, const int64_t* h_off, int32_t problem_type,
+                            int32_t reads, int32_t R, int32_t S, int32_t order, int32_t use_pt,
+                            int32_t best_semantics, const float* betas, uint64_t seed,
+                            uint64_t sweep_offset, const int64_t* s_off, int8_t* states,
+                            float* energies, int64_t* beta_idx, int8_t* best_state,
+                            float* best_energy, int32_t* best_sweeps, uint64_t* accepted,
+                            double* kernel_ms, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

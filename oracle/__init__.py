@@ -70,6 +70,9 @@ def load():
         lib.oracle_ais.argtypes = ([i32] * 3 + [vp, vp, vp] + [i32] * 3 + [dbl, i32, i32, u64] +
                                    [vp, vp, vp, vp, i32])
         lib.oracle_ais.restype = i32
+        lib.oracle_batched.argtypes = ([i32] + [vp] * 7 + [i32] * 7 + [vp, u64, u64] + [vp] * 7 +
+                                       [i32])
+        lib.oracle_batched.restype = i32
         _lib = lib
     return _lib
 
@@ -168,3 +171,27 @@ def ais(J, h, ptype, temps, states, *, ref=0, p=0.5, order=0, ordering=0, seed=0
                            _p(lr), _p(pm), _p(st), _p(lw), nthreads)
     assert rc == 0
     return st, lw
+
+
+def batched(packed, ptype, reads, R, S, betas, states, energies, *, order=0, use_pt=True,
+            best_sem=0, seed=0, sweep_offset=0, nthreads=1):
+    """Batched instances (see pysa_amd.batched.pack): states int8 flat,
+    energies float32 (count*reads, R).  Returns a dict of flat outputs."""
+    count = len(packed["n"])
+    st = np.ascontiguousarray(states, dtype=np.int8).copy()
+    en = np.ascontiguousarray(energies, dtype=np.float32).reshape(count * reads, R).copy()
+    bi = np.empty((count * reads, R), np.int64)
+    bs = np.empty(st.size // R, np.int8)
+    be = np.empty(count * reads, np.float32)
+    bw = np.empty(count * reads, np.int32)
+    acc = np.zeros(count * reads, np.uint64)
+    h = packed.get("h")
+    rc = load().oracle_batched(
+        count, _p(packed["n"]), _p(packed["ld"]), _p(packed["j_off"]), _p(packed["signs"]),
+        _p(packed["scale"]), _p(h), _p(packed["h_off"]) if h is not None else None, ptype, reads,
+        R, S, order, int(use_pt), best_sem, _p(np.ascontiguousarray(betas, np.float32)),
+        seed & (2**64 - 1), sweep_offset, _p(st), _p(en), _p(bi), _p(bs), _p(be), _p(bw), _p(acc),
+        nthreads)
+    assert rc == 0
+    return dict(states=st, energies=en, beta_idx=bi, best_state=bs, best_energy=be,
+                best_sweeps=bw, accepted=acc)

@@ -165,6 +165,13 @@ struct RunCfg {
   const int32_t* rep_perm;
   const double* rep_pt;
   int nthreads;
+  // Batched instances: couplings scale * signs (int8, leading dimension ld)
+  // with the field taken as the f32 value scale * (signs_i . s) + h_i, and
+  // read indices offset by read_base in the random streams.
+  const int8_t* signs = nullptr;
+  int ld = 0;
+  float scale = 1.f;
+  uint64_t read_base = 0;
 };
 
 // Negated energy change of flipping `pos` (ising.py:48-49, qubo.py:48-49):
@@ -188,7 +195,7 @@ double sweep_one(const RunCfg& c, const T* J, const T* h, T* s, T beta, int read
                  uint64_t* acc) {
   const int n = c.n;
   const uint32_t k0 = (uint32_t)c.seed, k1 = (uint32_t)(c.seed >> 32);
-  const uint64_t chain = (uint64_t)read * (uint64_t)c.R + (uint64_t)k;
+  const uint64_t chain = ((uint64_t)read + c.read_base) * (uint64_t)c.R + (uint64_t)k;
   const uint64_t sw = (uint64_t)sweep + c.sweep_offset;
   const size_t base = (((size_t)read * c.S + sweep) * c.R + k) * (size_t)n;
   Perm perm(n, philox10(ctr_perm(c.order == 2 ? 0 : chain, sw, c.order == 2 ? 3u : 2u), k0, k1));
@@ -213,7 +220,16 @@ double sweep_one(const RunCfg& c, const T* J, const T* h, T* s, T beta, int read
       }
       lr = dlog(u01(blk.v[pos & 3]));
     }
-    const T fh = (T)(row_dot<T>(J + (size_t)pos * n, s, n) + h[pos]);
+    T fh;
+    if (c.signs) {
+      long long dot = 0;  // exact integer count signs_pos . s
+      const int8_t* row = c.signs + (size_t)pos * c.ld;
+      for (int j = 0; j < n; ++j) dot += (long long)row[j] * (long long)s[j];
+      const float t1 = (float)dot * c.scale;
+      fh = (T)(t1 + (float)h[pos]);
+    } else {
+      fh = (T)(row_dot<T>(J + (size_t)pos * n, s, n) + h[pos]);
+    }
     const double dn = neg_delta<T>(c.ptype, s[pos], fh);
     if (dn >= 0 || lr < (double)beta * dn) {
       s[pos] = flipped<T>(c.ptype, s[pos]);
@@ -282,7 +298,8 @@ int metropolis(const RunCfg& c, const T* J, const T* h, const T* betas, const in
             if (c.rep_pt)
               lu = c.rep_pt[((size_t)read * S + s) * (R - 1) + k];
             else {
-              W4 w = philox10(ctr_pt((uint64_t)read, (uint64_t)s + c.sweep_offset, (uint32_t)k), k0, k1);
+              W4 w = philox10(ctr_pt((uint64_t)read + c.read_base, (uint64_t)s + c.sweep_offset,
+                                     (uint32_t)k), k0, k1);
               lu = dlog(u01(w.v[k & 3]));
             }
             sw = lu < (double)dd;
@@ -475,6 +492,38 @@ int oracle_metropolis(int n, int ptype, int dtype, const void* J, const void* h,
   return metropolis<double>(c, (const double*)J, (const double*)h, (const double*)betas, beta_sort,
                             states, (double*)energies, beta_idx, best_state, (double*)best_energy,
                             best_sweeps, log_omegas, accepted);
+}
+
+// Batched instances (the engine's pysa_amd_batched_device): instance k runs
+// the Metropolis / PT chain above with couplings scale[k] * signs_k and read
+// indices k*reads + read in the random streams.
+int oracle_batched(int count, const int32_t* ns, const int32_t* ld, const int64_t* j_off,
+                   const int8_t* signs, const float* scale, const float* h, const int64_t* h_off,
+                   int ptype, int reads, int R, int S, int order, int use_pt, int best_sem,
+                   const float* betas, uint64_t seed, uint64_t sweep_offset, int8_t* states,
+                   float* energies, int64_t* beta_idx, int8_t* best_state, float* best_energy,
+                   int32_t* best_sweeps, uint64_t* accepted, int nthreads) {
+  int64_t so = 0;
+  std::vector<int32_t> bsort(R);
+  for (int k = 0; k < R; ++k) bsort[k] = k;
+  for (int k = 0; k < count; ++k) {
+    const int n = ns[k];
+    RunCfg c = {n, ptype, reads, R, S, order, use_pt, 0, best_sem, seed, sweep_offset,
+                nullptr, nullptr, nullptr, nthreads};
+    c.signs = signs + j_off[k];
+    c.ld = ld[k];
+    c.scale = scale[k];
+    c.read_base = (uint64_t)k * reads;
+    std::vector<float> hk(n, 0.f);
+    if (h)
+      for (int i = 0; i < n; ++i) hk[i] = h[h_off[k] + i];
+    const size_t g = (size_t)k * reads;
+    metropolis<float>(c, nullptr, hk.data(), betas + (size_t)k * R, bsort.data(), states + so,
+                      energies + g * R, beta_idx + g * R, best_state + so / R, best_energy + g,
+                      best_sweeps + g, nullptr, accepted ? accepted + g : nullptr);
+    so += (int64_t)reads * R * n;
+  }
+  return 0;
 }
 
 int oracle_ais(int n, int ptype, int dtype, const void* J, const void* h, const void* temps, int nt,

@@ -40,6 +40,10 @@ _SIGNATURES = {
     "pysa_amd_ais": [_c_i64, _vp, _c_i32, _c_i32, _c_i32, _c_dbl, _c_i32, _c_i32, _c_u64,
                      _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
+    "pysa_amd_batched_device": [_c_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32,
+                                _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _vp, _c_u64, _c_u64,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl),
+                                _vp],
     "pysa_amd_malloc": [_c_u64, ctypes.POINTER(_vp)],
     "pysa_amd_free": [_vp],
     "pysa_amd_memcpy_h2d": [_vp, _vp, _c_u64],

@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -535,3 +536,112 @@ int pysa_amd_energy(int64_t handle, const int8_t* states, int64_t count, double*
 }
 
 }  // extern "C"
+
+extern "C" int pysa_amd_batched_device(int32_t count, const int32_t* n, const int32_t* ld,
+                                       const int64_t* j_off, const int8_t* signs,
+                                       const float* scale, const float* h, const int64_t* h_off,
+                                       int32_t problem_type, int32_t reads, int32_t R, int32_t S,
+                                       int32_t order, int32_t use_pt, int32_t best_semantics,
+                                       const float* betas, uint64_t seed, uint64_t sweep_offset,
+                                       const int64_t* s_off, int8_t* states, float* energies,
+                                       int64_t* beta_idx, int8_t* best_state, float* best_energy,
+                                       int32_t* best_sweeps, uint64_t* accepted, double* kernel_ms,
+                                       void* stream) {
+  g_err.clear();
+  if (count < 1 || reads < 1 || R < 1 || R > 16 || S < 1 || order < 0 || order > 2 ||
+      problem_type < 0 || problem_type > 1 || best_semantics < 0 || best_semantics > 1)
+    return fail(PYSA_AMD_EINVAL, "batched: count, reads >= 1, 1 <= R <= 16, S >= 1, order 0-2");
+  if (!n || !ld || !j_off || !signs || !scale || !betas || !s_off || !states || !energies ||
+      !beta_idx || !best_state || !best_energy || !best_sweeps || (h && !h_off))
+    return fail(PYSA_AMD_EINVAL, "batched: null array");
+  if ((int64_t)count * reads > 0x7fffffffLL) return fail(PYSA_AMD_EINVAL, "batched: too many reads");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int32_t> hn(count), hld(count);
+  std::vector<int64_t> hj(count), hs(count);
+  HIP_TRY(hipMemcpyAsync(hn.data(), n, sizeof(int32_t) * count, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hld.data(), ld, sizeof(int32_t) * count, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hj.data(), j_off, sizeof(int64_t) * count, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hs.data(), s_off, sizeof(int64_t) * count, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int max_n = 0;
+  int64_t acc_states = 0;
+  for (int k = 0; k < count; ++k) {
+    if (hn[k] < 1 || hn[k] > pysa::kBatchedMaxN)
+      return fail(PYSA_AMD_EINVAL, "batched: instance size out of [1, 2048]");
+    if (hld[k] < hn[k] || hld[k] % 16 != 0 || hj[k] < 0 || hj[k] % 16 != 0)
+      return fail(PYSA_AMD_EINVAL, "batched: ld must be >= n and a multiple of 16, j_off 16-aligned");
+    if (hs[k] != acc_states) return fail(PYSA_AMD_EINVAL, "batched: s_off must be the prefix sum of reads*R*n");
+    acc_states += (int64_t)reads * R * hn[k];
+    max_n = hn[k] > max_n ? hn[k] : max_n;
+  }
+  if (max_n > 1024 && R > 8) return fail(PYSA_AMD_EINVAL, "batched: R <= 8 when an instance has n > 1024");
+  // Largest instances first, so the short ones fill the tail of the launch.
+  const int items = count * reads;
+  std::vector<int32_t> om(items);
+  std::vector<int> idx(count);
+  for (int k = 0; k < count; ++k) idx[k] = k;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hn[a] > hn[b]; });
+  for (int q = 0, o = 0; q < count; ++q)
+    for (int r = 0; r < reads; ++r) om[o++] = idx[q] * reads + r;
+  void *d_om = nullptr, *d_best = nullptr;
+  HIP_TRY(hipMalloc(&d_om, sizeof(int32_t) * items));
+  if (best_semantics == 0 && hipMalloc(&d_best, (size_t)acc_states) != hipSuccess) {
+    (void)hipFree(d_om);
+    return fail(PYSA_AMD_EHIP, "batched: allocating best-state scratch");
+  }
+  struct Frees {
+    void* a;
+    void* b;
+    ~Frees() {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+    }
+  } frees{d_om, d_best};
+  HIP_TRY(hipMemcpyAsync(d_om, om.data(), sizeof(int32_t) * items, hipMemcpyHostToDevice, st));
+  if (accepted) HIP_TRY(hipMemsetAsync(accepted, 0, sizeof(uint64_t) * items, st));
+  pysa::BatchParams p;
+  p.count = count;
+  p.reads = reads;
+  p.R = R;
+  p.S = S;
+  p.order = order;
+  p.use_pt = use_pt ? 1 : 0;
+  p.best_sem = best_semantics;
+  p.ptype = problem_type;
+  p.k0 = (uint32_t)seed;
+  p.k1 = (uint32_t)(seed >> 32);
+  p.sweep_offset = sweep_offset;
+  p.inst_n = n;
+  p.ld = ld;
+  p.j_off = j_off;
+  p.signs = signs;
+  p.scale = scale;
+  p.h = h;
+  p.h_off = h_off;
+  p.betas = betas;
+  p.s_off = s_off;
+  p.order_map = (const int32_t*)d_om;
+  p.states = states;
+  p.energies = energies;
+  p.beta_idx = beta_idx;
+  p.best_state = best_state;
+  p.best_energy = best_energy;
+  p.best_sweeps = best_sweeps;
+  p.accepted = (unsigned long long*)accepted;
+  p.best_rep = (int8_t*)d_best;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, st);
+  hipError_t le = pysa::launch_batched(p, items, max_n, st);
+  (void)hipEventRecord(e1, st);
+  hipError_t se = (le == hipSuccess) ? hipEventSynchronize(e1) : le;
+  float t = 0.f;
+  if (se == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (le != hipSuccess) return fail(PYSA_AMD_EHIP, std::string("batched launch: ") + hipGetErrorString(le));
+  if (se != hipSuccess) return fail(PYSA_AMD_EHIP, std::string("batched run: ") + hipGetErrorString(se));
+  if (kernel_ms) *kernel_ms = t;
+  return 0;
+}

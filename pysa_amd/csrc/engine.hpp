@@ -65,4 +65,35 @@ hipError_t launch_plain_ais(const AISParams<T>& p, hipStream_t st);
 // Largest problem size the plain kernels hold in registers.
 constexpr int kPlainMaxN = 4096;
 
+// Many instances of different sizes in one launch (batched form).  Instance
+// k: n[k] spins, couplings scale[k] * S_k with S_k an int8 matrix (row-major,
+// leading dimension ld[k], a multiple of 16, at signs + j_off[k]), fields h +
+// h_off[k] (h may be null), ladder betas[k*R ..], states at s_off[k] =
+// sum_{j<k} reads*R*n[j]; best states at s_off[k]/R.
+struct BatchParams {
+  int count, reads, R, S, order, use_pt, best_sem, ptype;
+  uint32_t k0, k1;
+  uint64_t sweep_offset;
+  const int32_t* inst_n;
+  const int32_t* ld;
+  const int64_t* j_off;
+  const int8_t* signs;
+  const float* scale;
+  const float* h;
+  const int64_t* h_off;
+  const float* betas;
+  const int64_t* s_off;
+  const int32_t* order_map;  // workgroup -> (instance * reads + read), or null
+  int8_t* states;
+  float* energies;           // count*reads*R
+  int64_t* beta_idx;         // count*reads*R
+  int8_t* best_state;
+  float* best_energy;        // count*reads
+  int32_t* best_sweeps;      // count*reads
+  unsigned long long* accepted;  // count*reads or null
+  int8_t* best_rep;          // scratch, same layout as states
+};
+constexpr int kBatchedMaxN = 2048;
+hipError_t launch_batched(const BatchParams& p, int items, int max_n, hipStream_t st);
+
 }  // namespace pysa

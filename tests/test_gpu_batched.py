@@ -1,0 +1,92 @@
+"""Batched uneven instances in one launch (pysa_amd_batched_device) against
+the oracle's batched restatement, bit for bit."""
+
+import numpy as np
+import pytest
+
+import oracle
+from pysa_amd import _lib, batched
+
+pytestmark = pytest.mark.gpu
+
+
+def make_instances(rng, sizes, kind):
+    out = []
+    for n in sizes:
+        if kind == "sk":
+            A = np.triu(2 * rng.integers(0, 2, (n, n)) - 1, 1)
+            scale = np.float32(1 / np.sqrt(n))
+        else:
+            A = np.triu(rng.integers(-3, 4, (n, n)), 1)
+            scale = np.float32(rng.choice([1.0, 0.37, 2.5]))
+        out.append(((A + A.T).astype(np.int8), scale))
+    return out
+
+
+def run_both(inst, ptype, reads, R, S, order, best_sem, seed, with_h, rng, use_pt=True):
+    h = [rng.normal(0, 0.5, S_.shape[0]).astype(np.float32) for S_, _ in inst] if with_h else None
+    pk = batched.pack(inst, h)
+    total = int(reads * R * pk["n"].astype(np.int64).sum())
+    b = rng.integers(0, 2, total)
+    st = (b if ptype else 2 * b - 1).astype(np.int8)
+    betas = np.stack([np.geomspace(rng.uniform(1.5, 4.0), 0.2, R) for _ in inst]).astype(np.float32)
+    en = batched.initial_energies(pk, inst, st, reads, R, h)
+    want = oracle.batched(pk, ptype, reads, R, S, betas, st, en, order=order, use_pt=use_pt,
+                          best_sem=best_sem, seed=seed, nthreads=8)
+    got = batched.solve_batch(inst, problem_type="qubo" if ptype else "ising", reads=reads,
+                              num_replicas=R, num_sweeps=S, betas=betas,
+                              update_strategy={0: "sequential", 1: "random",
+                                               2: "random_shared"}[order],
+                              h=h, use_pt=use_pt, parallel=(best_sem == 0), seed=seed, states=st)
+    return got, want, pk
+
+
+def compare(got, want, pk, reads, R):
+    off = 0
+    for k, g in enumerate(got):
+        n = int(pk["n"][k])
+        sl = slice(k * reads, (k + 1) * reads)
+        assert np.array_equal(g["states"].reshape(-1), want["states"][off:off + reads * R * n]), k
+        assert np.array_equal(g["best_state"].reshape(-1),
+                              want["best_state"][off // R:off // R + reads * n]), k
+        off += reads * R * n
+        assert np.array_equal(g["energies"], want["energies"][sl]), k
+        assert np.array_equal(g["beta_idx"], want["beta_idx"][sl]), k
+        assert np.array_equal(g["best_energy"], want["best_energy"][sl]), k
+        assert np.array_equal(g["best_sweeps"], want["best_sweeps"][sl]), k
+        assert np.array_equal(g["accepted"], want["accepted"][sl]), k
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("order", [0, 1, 2])
+@pytest.mark.parametrize("best_sem", [0, 1])
+def test_batched_matches_oracle(ptype, order, best_sem):
+    rng = np.random.default_rng(100 + 6 * ptype + 2 * order + best_sem)
+    inst = make_instances(rng, [5, 33, 64, 100, 257, 31], "sk") + \
+        make_instances(rng, [48, 130], "int")
+    reads, R, S = 3, 5, 25
+    got, want, pk = run_both(inst, ptype, reads, R, S, order, best_sem, 7, ptype == 0, rng)
+    compare(got, want, pk, reads, R)
+
+
+@pytest.mark.parametrize("R", [1, 8, 16])
+def test_batched_replica_counts(R):
+    rng = np.random.default_rng(200 + R)
+    inst = make_instances(rng, [40, 90, 200], "sk")
+    got, want, pk = run_both(inst, 0, 2, R, 15, 1, 0, 3, False, rng, use_pt=(R > 1))
+    compare(got, want, pk, 2, R)
+
+
+def test_batched_large_instances():
+    """Instances above 1024 spins take the 32-slot register layout."""
+    rng = np.random.default_rng(300)
+    inst = make_instances(rng, [1100, 700, 2048], "sk")
+    got, want, pk = run_both(inst, 0, 2, 4, 4, 0, 0, 11, False, rng)
+    compare(got, want, pk, 2, 4)
+
+
+def test_batched_rejects_bad_sizes():
+    rng = np.random.default_rng(1)
+    inst = make_instances(rng, [40], "sk")
+    with pytest.raises(RuntimeError, match="R <= 16"):
+        batched.solve_batch(inst, num_replicas=17, num_sweeps=2)
