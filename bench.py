@@ -159,6 +159,98 @@ def run_form(wl, form_name, steps, warmup, seed):
     return out
 
 
+def batched_c4(sweeps=2000, count=2048, loop_instances=8, seed=1004):
+    """Sherrington-Kirkpatrick batch: `count` instances, n uniform in
+    [32, 768], J = +-1/sqrt(n) (int8 signs + scale), h = 0, 4 reads x 8
+    replicas each, T 0.3 -> 1.5, random order.  One batched launch with inputs
+    resident in HBM, against a per-instance launch loop (one engine call per
+    instance, as a loop over Solver objects would make) timed on the first
+    `loop_instances` instances."""
+    from pysa_amd import _lib, batched, instances
+
+    reads, R = 4, 8
+    inst = instances.sk_batch(count, 32, 768, seed)
+    pk = batched.pack(inst)
+    ns = pk["n"].astype(np.int64)
+    total = int(reads * R * ns.sum())
+    rng = np.random.default_rng(seed)
+    st = (2 * rng.integers(0, 2, total) - 1).astype(np.int8)
+    en = batched.initial_energies(pk, inst, st, reads, R)
+    betas = np.tile(ladder(R), (count, 1))
+    s_off = np.concatenate([[0], np.cumsum(reads * R * ns)[:-1]]).astype(np.int64)
+    D = _lib.DeviceArray
+    dev = {k: D.from_numpy(v) for k, v in pk.items()}
+    d_betas, d_soff = D.from_numpy(betas), D.from_numpy(s_off)
+    d_st0, d_en0 = D.from_numpy(st), D.from_numpy(en)
+    d_st, d_en = D(st.shape, np.int8), D(en.shape, np.float32)
+    d_bi = D((count * reads, R), np.int64)
+    d_bs = D(total // R, np.int8)
+    d_be, d_bw = D(count * reads, np.float32), D(count * reads, np.int32)
+    d_acc = D(count * reads, np.uint64)
+    L = _lib
+
+    def launch(sd):
+        d_st.copy_from(d_st0)
+        d_en.copy_from(d_en0)
+        kms = L._c_dbl(0.0)
+        L.check(L.load().pysa_amd_batched_device(
+            count, dev["n"].ptr, dev["ld"].ptr, dev["j_off"].ptr, dev["signs"].ptr,
+            dev["scale"].ptr, None, None, 0, reads, R, sweeps, 1, 1, 0, d_betas.ptr, sd, 0,
+            d_soff.ptr, d_st.ptr, d_en.ptr, d_bi.ptr, d_bs.ptr, d_be.ptr, d_bw.ptr, d_acc.ptr,
+            L.ctypes.byref(kms), None))
+        return kms.value
+
+    launch(1)
+    log("[bench] batched C4: warmup launch done")
+    kms = launch(2)
+    updates = float(total) * sweeps
+    acc = int(d_acc.numpy().sum())
+    log(f"[bench] batched C4: {kms:.1f} ms device")
+    # Per-instance loop: one plain-form engine call per instance (the batch
+    # as a caller's loop would run it), first `loop_instances` instances.
+    loop_ms, loop_upd = 0.0, 0.0
+    for k in range(loop_instances):
+        S_k, sc = inst[k]
+        n = S_k.shape[0]
+        prob = L.Problem((S_k.astype(np.float32) * sc).astype(np.float32),
+                         np.zeros(n, np.float32), "ising", np.float32)
+        o = int(s_off[k])
+        st_k = D.from_numpy(st[o:o + reads * R * n])
+        en_k = D.from_numpy(en[k * reads:(k + 1) * reads])
+        bi_k, bs_k = D((reads, R), np.int64), D((reads, n), np.int8)
+        be_k, bw_k = D(reads, np.float32), D(reads, np.int32)
+        b_k = D.from_numpy(ladder(R))
+        srt = D.from_numpy(np.argsort(ladder(R)).astype(np.int32))
+        km = L._c_dbl(0.0)
+        t0 = time.perf_counter()
+        L.check(L.load().pysa_amd_metropolis_device(
+            prob.handle, reads, R, sweeps, 1, 1, 0, 0, b_k.ptr, srt.ptr, 3, 0, L.FORM_PLAIN, 0,
+            None, None, None, st_k.ptr, en_k.ptr, bi_k.ptr, bs_k.ptr, be_k.ptr, bw_k.ptr, None,
+            None, L.ctypes.byref(km), None))
+        loop_ms += 1e3 * (time.perf_counter() - t0)
+        loop_upd += float(n) * R * reads * sweeps
+        for a in (st_k, en_k, bi_k, bs_k, be_k, bw_k, b_k, srt):
+            a.free()
+        prob.close()
+    log(f"[bench] per-instance loop over {loop_instances} instances: {loop_ms:.1f} ms wall")
+    for a in list(dev.values()) + [d_betas, d_soff, d_st0, d_en0, d_st, d_en, d_bi, d_bs, d_be,
+                                   d_bw, d_acc]:
+        a.free()
+    return {
+        "workload": f"{count} SK instances, n uniform in [32, 768] (sum {int(ns.sum())}), "
+                    f"J = +-1/sqrt(n) int8 signs x scale, h = 0, {reads} reads x {R} replicas x "
+                    f"{sweeps} sweeps, T 0.3->1.5, random order, PT",
+        "updates": updates,
+        "kernel_ms": kms,
+        "value": updates / (kms * 1e-3),
+        "unit": "spin_updates/s",
+        "acceptance": acc / updates,
+        "per_instance_loop": {"instances": loop_instances, "updates_per_s": loop_upd / (loop_ms * 1e-3),
+                              "note": "wall time of one plain-form engine call per instance"},
+        "batched_over_loop": (updates / (kms * 1e-3)) / (loop_upd / (loop_ms * 1e-3)),
+    }
+
+
 def cpu_baseline(target_s=12.0, threads=None):
     """The oracle's C restatement of the reference algorithm (O(N) coupling
     row dot per proposal, replicas in parallel as numba.prange does) on the
@@ -203,6 +295,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=5000)
     ap.add_argument("--forms", default="auto", help="auto | plain | blocked | both")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances line")
     ap.add_argument("--seed", type=int, default=1003)
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain form when it is the comparison, not the headline")
@@ -265,6 +358,8 @@ def main():
     }
     if "plain" in res and "blocked" in res:
         line["blocked_over_plain"] = res["blocked"]["value"] / res["plain"]["value"]
+    if not args.no_batched:
+        line["batched_c4"] = batched_c4()
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline()

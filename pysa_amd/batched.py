@@ -60,7 +60,7 @@ def initial_energies(packed, instances, states, reads, R, h=None):
         st = states[off:off + reads * R * n].reshape(reads * R, n).astype(np.float64)
         off += reads * R * n
         J = np.asarray(S, dtype=np.float64) * float(np.float32(scale))
-        e = 0.5 * np.einsum("ki,ij,kj->k", st, J, st)
+        e = 0.5 * np.sum((st @ J) * st, axis=1)
         if h is not None:
             e = e + st @ np.asarray(h[k], dtype=np.float32).astype(np.float64)
         out.append(e.astype(np.float32).reshape(reads, R))
