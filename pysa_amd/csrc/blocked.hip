@@ -152,20 +152,37 @@ __device__ __forceinline__ void spins_i8(uint32_t x, int ptype, uint32_t (&d)[4]
   }
 }
 
-// Certified acceptance test for dn < 0: returns det_log(u) < beta*dn (exact
-// product in f64) for u = (w + 0.5) 2^-32.  Decided in single precision when
-// the gap exceeds the error bound of the hardware logarithm plus the rounding
-// of the product to f32; otherwise (rarely) in exact f64, under a branch the
-// wave skips when no lane needs it.  `need` masks lanes whose answer is used.
-__device__ __forceinline__ bool log_less(uint32_t w, float beta, float dn32, bool need) {
-  const float x32 = beta * dn32;
+// Certified acceptance thresholds.  PySA accepts a proposal iff
+// dn >= 0 or det_log(u) < beta*dn (product exact in f64), with
+// u = (w + 0.5) 2^-32.  For beta > 0 that is dn > q with q = det_log(u)/beta
+// (q < 0).  `accept_bounds` brackets q from the hardware logarithm:
+// dn > hi certainly accepts, dn < lo certainly rejects, and only a dn inside
+// [lo, hi] (a relative width of a few 1e-6) needs the exact test.  The bound
+// on |L - log(u)| covers the rounding of u to f32, the hardware log2 and the
+// scaling by ln 2; the bracket adds the rounding of q = L/beta and of hi, lo.
+// beta <= 0 accepts every proposal (hi = lo = -inf).
+__device__ __forceinline__ void accept_bounds(uint32_t w, float beta, float inv_b, float& lo,
+                                              float& hi) {
+  if (!(beta > 0.f)) {
+    lo = hi = -INFINITY;
+    return;
+  }
+  if (!(inv_b < 1e30f)) {  // beta too small to scale: every decision goes exact
+    lo = -INFINITY;
+    hi = INFINITY;
+    return;
+  }
   const float uf = ((float)w + 0.5f) * 2.3283064365386963e-10f;
   const float L = __builtin_amdgcn_logf(uf) * 0.69314718055994531f;
-  const float m = 3.9e-6f * fabsf(L) + 9.6e-7f + 2.4e-7f * fabsf(x32);
-  bool r = L < x32 - m;
-  const bool sure = r || (L > x32 + m);
-  if (__builtin_expect(need && !sure, 0)) r = log_u32(w) < (double)beta * (double)dn32;
-  return r;
+  const float q = L * inv_b;
+  const float m = (3.9e-6f * fabsf(L) + 9.6e-7f) * inv_b * 1.001f + 3.6e-7f * fabsf(q);
+  hi = q + m;
+  lo = q - m;
+}
+
+// The exact test (pysa/ising.py:52) for the rare dn inside the bracket.
+__device__ __noinline__ bool accept_exact(uint32_t w, float beta, float dn) {
+  return dn >= 0.f || log_u32(w) < (double)beta * (double)dn;
 }
 
 template <int NW>
@@ -365,6 +382,7 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
     const uint64_t sw = (uint64_t)s + p.sweep_offset;
     const float beta = act ? L.rb[q] : 0.f;
     const float cu = (p.ptype == 0) ? 2.f : 1.f;  // |dn / (F + h)|
+    const float inv_b = (beta > 0.f) ? 1.0f / beta : 0.f;
     double dsum = 0.0;
     uint32_t nacc = 0;
     for (int c = 0; c < nblk; ++c) {
@@ -446,17 +464,18 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
       // of J^T[blk][blk].
       {
         const int tq = q >> 5, jq = q & 31, r0 = a * RPL;
-        float f[RPL], hv[RPL];
+        // fp = F + h of this lane's RPL rows
+        float fp[RPL];
 #pragma unroll
         for (int rl = 0; rl < RPL; ++rl) {
           float t = 0.f;
 #pragma unroll
           for (int ww = 0; ww < KW; ++ww) t += L.part[ww][tq][jq * 33 + r0 + rl];
-          f[rl] = t;
-          hv[rl] = L.hb[r0 + rl];
+          fp[rl] = t + L.hb[r0 + rl];
         }
         const uint32_t all = L.bb[buf][q];
         uint32_t mb = (RPL == 32) ? all : ((all >> r0) & ((1u << RPL) - 1u));
+        const uint32_t mb0 = mb;
         uint32_t uw[RPL];
 #pragma unroll
         for (int m = 0; m < RPL / 4; ++m) {
@@ -466,34 +485,44 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
           uw[4 * m + 2] = blk.z;
           uw[4 * m + 3] = blk.w;
         }
+        // Acceptance brackets of the lane's rows, off the sequential chain.
+        float tlo[RPL], thi[RPL];
+#pragma unroll
+        for (int rl = 0; rl < RPL; ++rl) {
+          accept_bounds(uw[rl], beta, inv_b, tlo[rl], thi[rl]);
+          if (pos0 + r0 + rl >= n) tlo[rl] = thi[rl] = INFINITY;  // padding: never taken
+        }
+        float dpart = 0.f;  // this lane's accepted energy changes (exact for integer couplings)
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
-          if (pos0 + k < n) {
+          {
             const int ak = k / RPL, kl = k % RPL;
             const bool up = (mb >> kl) & 1u;
             // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
             const float sg = up ? cu : -cu;
-            const float dn32 = (f[kl] + hv[kl]) * sg;
+            const float dn = fp[kl] * sg;
             const bool own = act && (a == ak);
-            // every lane evaluates the test; only the owner of row k keeps it
-            const bool lt = log_less(uw[kl], beta, dn32, own & (dn32 < 0.f));
-            const bool take = own & ((dn32 >= 0.f) | lt);
+            bool take = dn > thi[kl];
+            if (__builtin_expect(own & !take & !(dn < tlo[kl]), 0))
+              take = accept_exact(uw[kl], beta, dn);
+            take = take & own;
             mb ^= (uint32_t)take << kl;
-            dsum += take ? -(double)dn32 : 0.0;
-            nacc += take ? 1u : 0u;
+            dpart += take ? -dn : 0.f;
             const float dS = group_bcast<LPR>(take ? -sg : 0.f, ak);
             const float4* jr = reinterpret_cast<const float4*>(&L.jb[k * kB + r0]);
 #pragma unroll
             for (int m = 0; m < RPL / 4; ++m) {
               const float4 j4 = jr[m];
               // dS is 0, +-1 or +-2, so the fused form rounds like mul + add
-              f[4 * m] = __builtin_fmaf(j4.x, dS, f[4 * m]);
-              f[4 * m + 1] = __builtin_fmaf(j4.y, dS, f[4 * m + 1]);
-              f[4 * m + 2] = __builtin_fmaf(j4.z, dS, f[4 * m + 2]);
-              f[4 * m + 3] = __builtin_fmaf(j4.w, dS, f[4 * m + 3]);
+              fp[4 * m] = __builtin_fmaf(j4.x, dS, fp[4 * m]);
+              fp[4 * m + 1] = __builtin_fmaf(j4.y, dS, fp[4 * m + 1]);
+              fp[4 * m + 2] = __builtin_fmaf(j4.z, dS, fp[4 * m + 2]);
+              fp[4 * m + 3] = __builtin_fmaf(j4.w, dS, fp[4 * m + 3]);
             }
           }
         }
+        dsum += (double)dpart;
+        nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
         if (RPL == 32)
           L.bb[buf][q] = mb;
         else if (RPL == 16)
