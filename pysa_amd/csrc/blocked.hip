@@ -131,24 +131,26 @@ __device__ __forceinline__ uint32_t block_of(uint32_t h0, uint32_t h1) {
 }
 
 // Eight spins (bits of x) as bf16 pairs: +1/-1 (ising) or 1/0 (qubo).
-__device__ __forceinline__ void spins_bf16(uint32_t x, int ptype, uint32_t (&d)[4]) {
+// Eight spins (bits of x) as bf16 pairs: (sp * mul) ^ xr with sp the bits
+// spread to positions 0 and 16; ising (mul, xr) = (0x8000, 0xBF80BF80) gives
+// +1 / -1 (an up spin clears the sign bit of -1), qubo (0x3F80, 0) gives 1 / 0.
+__device__ __forceinline__ void spins_bf16(uint32_t x, uint32_t mul, uint32_t xr,
+                                           uint32_t (&d)[4]) {
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const uint32_t t = (x >> (2 * e)) & 3u;
-    const uint32_t sp = ((t | (t << 15)) & 0x10001u);  // bit0 -> 0, bit1 -> 16
-    if (ptype == 0)
-      d[e] = 0xBF80BF80u ^ (sp << 15);  // up clears the sign bit
-    else
-      d[e] = sp * 0x3F80u;
+    const uint32_t sp = (t | (t << 15)) & 0x10001u;  // bit0 -> 0, bit1 -> 16
+    d[e] = (sp * mul) ^ xr;
   }
 }
-// Sixteen spins (bits of x) as int8 +1/-1 (ising) or 1/0 (qubo).
-__device__ __forceinline__ void spins_i8(uint32_t x, int ptype, uint32_t (&d)[4]) {
+// Sixteen spins (bits of x) as int8: ising (mul, xr) = (0xFE, 0xFFFFFFFF)
+// gives +1 / -1 per byte, qubo (1, 0) gives 1 / 0.
+__device__ __forceinline__ void spins_i8(uint32_t x, uint32_t mul, uint32_t xr, uint32_t (&d)[4]) {
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const uint32_t t = (x >> (4 * e)) & 15u;
     const uint32_t sp = (t * 0x00204081u) & 0x01010101u;  // bit b -> byte b
-    d[e] = (ptype == 0) ? (0xFFFFFFFFu - sp * 0xFEu) : sp;
+    d[e] = (sp * mul) ^ xr;
   }
 }
 
@@ -345,6 +347,9 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
   // block, and its J[blk][blk] / h[blk] share, are requested right after the
   // current block's products so their latency hides behind the proposals.
   constexpr int PF = 4;
+  // Operand expansion constants of the spin encoding (see spins_bf16 / spins_i8).
+  const uint32_t emul = I8 ? (p.ptype == 0 ? 0xFEu : 1u) : (p.ptype == 0 ? 0x8000u : 0x3F80u);
+  const uint32_t exr = I8 ? (p.ptype == 0 ? 0xFFFFFFFFu : 0u) : (p.ptype == 0 ? 0xBF80BF80u : 0u);
   constexpr int NST = I8 ? 1 : 2;  // fragments per K step of one block column
   constexpr int NJB = (kB * kB + 64 * KW - 1) / (64 * KW);
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
@@ -421,8 +426,8 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
           const uint32_t ha = get_half<NW>(sa, j), hb = get_half<NW>(sb, j);
           if (I8) {
             uint32_t da[4], db[4];
-            spins_i8(ha, p.ptype, da);
-            spins_i8(hb, p.ptype, db);
+            spins_i8(ha, emul, exr, da);
+            spins_i8(hb, emul, exr, db);
             i32x4 ba, bb;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -436,8 +441,8 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
 #pragma unroll
             for (int st = 0; st < NST; ++st) {
               uint32_t da[4], db[4];
-              spins_bf16((ha >> (8 * st)) & 0xFFu, p.ptype, da);
-              spins_bf16((hb >> (8 * st)) & 0xFFu, p.ptype, db);
+              spins_bf16((ha >> (8 * st)) & 0xFFu, emul, exr, da);
+              spins_bf16((hb >> (8 * st)) & 0xFFu, emul, exr, db);
               u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
               const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bb = __builtin_bit_cast(bf16x8, ub);
 #pragma unroll
