@@ -73,6 +73,15 @@ __host__ __device__ inline float bf16_to_f(uint16_t b) {
   return f;
 }
 
+// Row sums of J in f64, rounded once (QUBO fields from +-1 products).
+__global__ void rowsum_kernel(const float* __restrict__ J, int n, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < n; ++j) s += (double)J[(size_t)i * n + j];
+  out[i] = (float)s;
+}
+
 // Fragment (c, step, piece) of the bf16 form: lane l holds
 // piece(J[32c + l%32][16*step + 8*(l/32) + e]), e < 8.
 __global__ void pack_bf16_kernel(const float* __restrict__ J, int n, int nblk, int np,
@@ -130,19 +139,19 @@ __device__ __forceinline__ uint32_t block_of(uint32_t h0, uint32_t h1) {
   return (h0 & 0xFFu) | ((h1 & 0xFFu) << 8) | ((h0 >> 8) << 16) | ((h1 >> 8) << 24);
 }
 
-// Eight spins (bits of x) as bf16 pairs: +1/-1 (ising) or 1/0 (qubo).
-// Eight spins (bits of x) as bf16 pairs: (sp * mul) ^ xr with sp the bits
-// spread to positions 0 and 16; ising (mul, xr) = (0x8000, 0xBF80BF80) gives
-// +1 / -1 (an up spin clears the sign bit of -1), qubo (0x3F80, 0) gives 1 / 0.
-__device__ __forceinline__ void spins_bf16(uint32_t x, uint32_t mul, uint32_t xr,
-                                           uint32_t (&d)[4]) {
+// Sixteen spins (bits 0-15 of y: K step 0 in bits 0-7, step 1 in 8-15) as
+// bf16 +1 / -1 operand words, two spins per dword.  nz = ~(y | y << 15) puts
+// the complement of spin 2e at bit 2e and of spin 2e+1 at bit 2e+16; a shift
+// by 15 - 2e (step 0) or 7 - 2e (step 1) moves them to the sign bits 15 and
+// 31, which a bit-field insert drops into -1.0 (0xBF80).  (The blocked form
+// always multiplies +-1 spins; QUBO fields are recovered from J.s as
+// (J.s + J.1) / 2.)
+template <int ST>
+__device__ __forceinline__ void spins_bf16(uint32_t nz, uint32_t (&d)[4]) {
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const uint32_t t = (x >> (2 * e)) & 3u;
-    const uint32_t sp = (t | (t << 15)) & 0x10001u;  // bit0 -> 0, bit1 -> 16
-    d[e] = (sp * mul) ^ xr;
-  }
+  for (int e = 0; e < 4; ++e) d[e] = ((nz << (15 - 8 * ST - 2 * e)) & 0x80008000u) | 0x3F803F80u;
 }
+__device__ __forceinline__ uint32_t spins_nz(uint32_t y) { return ~(y | (y << 15)); }
 // Sixteen spins (bits of x) as int8: ising (mul, xr) = (0xFE, 0xFFFFFFFF)
 // gives +1 / -1 per byte, qubo (1, 0) gives 1 / 0.
 __device__ __forceinline__ void spins_i8(uint32_t x, uint32_t mul, uint32_t xr, uint32_t (&d)[4]) {
@@ -228,6 +237,7 @@ struct BParams {
   const void* frag;
   const float* Jt;  // f32 J^T, row j = coefficients J[:][j] (in-block updates)
   const float* h;
+  const float* rowsum;  // n row sums of J (QUBO with bf16 operands), else null
   const float* betas;
   const int* bsort;
   int8_t* spins;
@@ -250,6 +260,7 @@ struct GroupLds {
   uint32_t bb[2][64];          // block spin bits per replica (double-buffered by block parity)
   float jb[kB * kB];           // jb[i*32 + k] = J^T[pos0+i][pos0+k] = J[pos0+k][pos0+i]
   float hb[kB];
+  float rsb[kB];  // row sums J[pos0+i] . 1 of the block rows (QUBO, bf16 operands)
   float E[64];
   float rb[64];   // beta of each replica
   int bi[64];     // slot -> replica within its read, at g*R + slot
@@ -348,13 +359,13 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
   // current block's products so their latency hides behind the proposals.
   constexpr int PF = 4;
   // Operand expansion constants of the spin encoding (see spins_bf16 / spins_i8).
-  const uint32_t emul = I8 ? (p.ptype == 0 ? 0xFEu : 1u) : (p.ptype == 0 ? 0x8000u : 0x3F80u);
-  const uint32_t exr = I8 ? (p.ptype == 0 ? 0xFFFFFFFFu : 0u) : (p.ptype == 0 ? 0xBF80BF80u : 0u);
+  const uint32_t emul = p.ptype == 0 ? 0xFEu : 1u;
+  const uint32_t exr = p.ptype == 0 ? 0xFFFFFFFFu : 0u;
   constexpr int NST = I8 ? 1 : 2;  // fragments per K step of one block column
   constexpr int NJB = (kB * kB + 64 * KW - 1) / (64 * KW);
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
   Frag ring[PF][NST][NP];
-  float jb_reg[NJB], h_reg = 0.f;
+  float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
   const int jn = max(0, min(kb, nblk - c0));
   auto frag_ptr = [&](int c, int j, int st, int pc) -> const Frag* {
     if (I8) return reinterpret_cast<const Frag*>(p.frag) + ((size_t)c * nblk + c0 + j) * 64 + lane;
@@ -380,6 +391,7 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
                        ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
     }
     h_reg = (tid < kB && pos0 + tid < n) ? p.h[pos0 + tid] : 0.f;
+    if (!I8 && p.rowsum) rs_reg = (tid < kB && pos0 + tid < n) ? p.rowsum[pos0 + tid] : 0.f;
   };
   prefetch_block(0);
 
@@ -398,7 +410,10 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
         const int e = tid + e2 * 64 * KW;
         if (e < kB * kB) L.jb[e] = jb_reg[e2];
       }
-      if (tid < kB) L.hb[tid] = h_reg;
+      if (tid < kB) {
+        L.hb[tid] = h_reg;
+        L.rsb[tid] = rs_reg;
+      }
       // ---- phase 1b: the owner publishes the block's spins of all 64 replicas
       if (w == oc) {
         const int j = c - c0;
@@ -438,11 +453,17 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
             iacc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, ba, iacc0, 0, 0, 0);
             iacc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, bb, iacc1, 0, 0, 0);
           } else {
+            const uint32_t nza = spins_nz(ha), nzb = spins_nz(hb);
 #pragma unroll
             for (int st = 0; st < NST; ++st) {
               uint32_t da[4], db[4];
-              spins_bf16((ha >> (8 * st)) & 0xFFu, emul, exr, da);
-              spins_bf16((hb >> (8 * st)) & 0xFFu, emul, exr, db);
+              if (st == 0) {
+                spins_bf16<0>(nza, da);
+                spins_bf16<0>(nzb, db);
+              } else {
+                spins_bf16<1>(nza, da);
+                spins_bf16<1>(nzb, db);
+              }
               u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
               const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bb = __builtin_bit_cast(bf16x8, ub);
 #pragma unroll
@@ -476,6 +497,8 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
           float t = 0.f;
 #pragma unroll
           for (int ww = 0; ww < KW; ++ww) t += L.part[ww][tq][jq * 33 + r0 + rl];
+          // QUBO with bf16 operands: the product used +-1 spins, F = (J.s + J.1) / 2
+          if (!I8 && p.ptype == 1) t = 0.5f * (t + L.rsb[r0 + rl]);
           fp[rl] = t + L.hb[r0 + rl];
         }
         const uint32_t all = L.bb[buf][q];
@@ -687,6 +710,7 @@ hipError_t launch_np(const BParams& p, int nw, int kw, int groups, hipStream_t s
 void blocked_release(BlockedProblem& b) {
   if (b.packed) (void)hipFree(b.packed);
   if (b.scratch) (void)hipFree(b.scratch);
+  if (b.rowsum) (void)hipFree(b.rowsum);
   b = BlockedProblem();
 }
 
@@ -777,6 +801,19 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     }
     b.packed_kind = kind;
   }
+  if (!i8 && ptype == 1 && !b.rowsum) {
+    if (hipMalloc(&b.rowsum, sizeof(float) * n) != hipSuccess) {
+      b.rowsum = nullptr;
+      *why = "allocating row sums";
+      return PYSA_AMD_EHIP;
+    }
+    hipLaunchKernelGGL(rowsum_kernel, dim3((n + 255) / 256), dim3(256), 0, st, (const float*)dJ, n,
+                       b.rowsum);
+    if (hipGetLastError() != hipSuccess) {
+      *why = "row sums";
+      return PYSA_AMD_EHIP;
+    }
+  }
   const size_t cnt = (size_t)reads * R;
   const size_t need = cnt * n + cnt * 4 + cnt * 4;
   if (b.scratch_bytes < need) {
@@ -807,6 +844,7 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   p.frag = b.packed;
   p.Jt = (const float*)dJt;
   p.h = (const float*)dh;
+  p.rowsum = (!i8 && ptype == 1) ? b.rowsum : nullptr;
   p.betas = (const float*)betas;
   p.bsort = bsort;
   p.spins = states;

@@ -19,6 +19,7 @@ struct BlockedProblem {
   int last_kind = -1;    // encoding used by the last run
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  float* rowsum = nullptr;  // J . 1 per row (QUBO with bf16 operands)
 };
 
 void blocked_release(BlockedProblem& b);
