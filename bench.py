@@ -251,6 +251,38 @@ def batched_c4(sweeps=2000, count=2048, loop_instances=8, seed=1004):
     }
 
 
+def ais_c5(nt=10000, chains=4096, seed=1005):
+    """Annealed importance sampling on the RBM-shaped Ising model: 784 visible
+    + 500 hidden spins, W ~ N(0, 0.05^2), biases ~ N(0, 0.1^2), float32,
+    `chains` chains over `nt` linear betas to 1, uniform reference, random
+    order, log-weights in f64 (pysa/sa.py:432-553)."""
+    from pysa_amd import _lib, instances
+
+    P = instances.rbm_ising(784, 500, seed=seed)
+    n = P.shape[0]
+    J = P.copy()
+    np.fill_diagonal(J, 0)
+    h = np.diag(P).astype(np.float32).copy()
+    prob = _lib.Problem(J.astype(np.float32), h, "ising", np.float32)
+    temps = np.array([(t + 1) / nt for t in range(nt)], dtype=np.float32)
+    rng = np.random.default_rng(seed)
+    st = (2 * rng.integers(0, 2, (chains, n)) - 1).astype(np.int8)
+    lw = np.empty(chains, np.float64)
+    kms = _lib._c_dbl(0.0)
+    _lib.check(_lib.load().pysa_amd_ais(prob.handle, _lib.ptr(temps), nt, chains, 0, 0.5, 1, 0,
+                                        seed, None, None, _lib.ptr(st), _lib.ptr(lw),
+                                        _lib.ctypes.byref(kms)))
+    prob.close()
+    updates = float(n) * nt * chains
+    m = lw.max()
+    logz = m + math.log(np.exp(lw - m).sum()) - math.log(chains) + n * math.log(2)
+    log(f"[bench] AIS C5: {kms.value:.1f} ms device")
+    return {"workload": f"AIS, RBM-shaped Ising 784+500 float32, {chains} chains x {nt} betas, "
+                        "uniform reference, random order",
+            "updates": updates, "kernel_ms": kms.value, "value": updates / (kms.value * 1e-3),
+            "unit": "spin_updates/s", "log_z": logz, "form": "plain"}
+
+
 def cpu_baseline(target_s=12.0, threads=None):
     """The oracle's C restatement of the reference algorithm (O(N) coupling
     row dot per proposal, replicas in parallel as numba.prange does) on the
@@ -296,6 +328,7 @@ def main():
     ap.add_argument("--forms", default="auto", help="auto | plain | blocked | both")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-instances line")
+    ap.add_argument("--no-ais", action="store_true", help="skip the AIS line")
     ap.add_argument("--seed", type=int, default=1003)
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain form when it is the comparison, not the headline")
@@ -360,6 +393,8 @@ def main():
         line["blocked_over_plain"] = res["blocked"]["value"] / res["plain"]["value"]
     if not args.no_batched:
         line["batched_c4"] = batched_c4()
+    if not args.no_ais:
+        line["ais_c5"] = ais_c5()
     if not args.no_cpu:
         log("[bench] CPU baseline ...")
         line["cpu_baseline"] = cpu_baseline()
