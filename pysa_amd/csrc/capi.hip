@@ -53,6 +53,8 @@ struct Problem {
   bool symmetric = true;
   void* dJ = nullptr;   // row-major J
   void* dJt = nullptr;  // J transposed (== dJ when J is exactly symmetric)
+  void* dJp = nullptr;  // rows of dJt padded with zeros to ldp (== dJt when ldp == n)
+  int ldp = 0;  // a multiple of 256
   void* dh = nullptr;
   hipStream_t stream = nullptr;
   std::mutex mu;  // calls on one problem are serialised
@@ -62,6 +64,7 @@ struct Problem {
   ~Problem() {
     (void)hipSetDevice(device);
     pysa::blocked_release(blocked);
+    if (dJp && dJp != dJt) (void)hipFree(dJp);
     if (dJt && dJt != dJ) (void)hipFree(dJt);
     if (dJ) (void)hipFree(dJ);
     if (dh) (void)hipFree(dh);
@@ -160,6 +163,8 @@ int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
   p.k1 = (uint32_t)(a.seed >> 32);
   p.sweep_offset = a.sweep_offset;
   p.Jt = (const T*)P->dJt;
+  p.Jp = (const T*)P->dJp;
+  p.ldp = P->ldp;
   p.h = (const T*)P->dh;
   p.betas = (const T*)a.betas;
   p.bsort = a.bsort;
@@ -314,6 +319,21 @@ int pysa_amd_problem_create(int32_t n, int32_t problem_type, int32_t dtype, cons
     P->symmetric = sym;
     if (sym) P->dJt = P->dJ;
   }
+  // Row-padded copy for the plain kernels' 16-byte row loads: whole
+  // 256-element chunks per row.
+  if (e == hipSuccess) {
+    P->ldp = ((n + 255) / 256) * 256;
+    if (P->ldp == n) {
+      P->dJp = P->dJt;
+    } else {
+      const size_t bytes = (size_t)n * P->ldp * es;
+      e = hipMalloc(&P->dJp, bytes);
+      if (e == hipSuccess) e = hipMemset(P->dJp, 0, bytes);
+      if (e == hipSuccess)
+        e = hipMemcpy2D(P->dJp, (size_t)P->ldp * es, P->dJt, (size_t)n * es, (size_t)n * es, n,
+                        hipMemcpyDeviceToDevice);
+    }
+  }
   if (e != hipSuccess) {
     std::string m = std::string("problem upload: ") + hipGetErrorString(e);
     delete P;
@@ -455,14 +475,16 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   if (P->dtype == 0) {
     pysa::AISParams<float> a = {(int)n, P->ptype, chains, nt, order, reference, ordering,
                                 (uint32_t)seed, (uint32_t)(seed >> 32), lp, l1p,
-                                (const float*)P->dJt, (const float*)P->dh, (const float*)dT,
+                                (const float*)P->dJt, (const float*)P->dJp, P->ldp,
+                                (const float*)P->dh, (const float*)dT,
                                 (const double*)dlr, (const int*)dperm, (int8_t*)dS, (float*)dF,
                                 (double*)dE, (double*)dW};
     le = pysa::launch_plain_ais<float>(a, st);
   } else {
     pysa::AISParams<double> a = {(int)n, P->ptype, chains, nt, order, reference, ordering,
                                  (uint32_t)seed, (uint32_t)(seed >> 32), lp, l1p,
-                                 (const double*)P->dJt, (const double*)P->dh, (const double*)dT,
+                                 (const double*)P->dJt, (const double*)P->dJp, P->ldp,
+                                 (const double*)P->dh, (const double*)dT,
                                  (const double*)dlr, (const int*)dperm, (int8_t*)dS, (double*)dF,
                                  (double*)dE, (double*)dW};
     le = pysa::launch_plain_ais<double>(a, st);

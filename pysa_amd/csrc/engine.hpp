@@ -17,6 +17,8 @@ struct PTParams {
   uint32_t k0, k1;
   uint64_t sweep_offset;
   const T* Jt;       // n*n, row p holds the coefficients J[i][p] (column p of J)
+  const T* Jp;       // the rows of Jt padded with zeros to ldp (a multiple of 256)
+  int ldp;
   const T* h;        // n
   const T* betas;    // R, slot order (slot 0 = coldest)
   const int* bsort;  // R, slots sorted by ascending beta (log-omega)
@@ -44,6 +46,8 @@ struct AISParams {
   uint32_t k0, k1;
   double lp, l1p;  // log p, log(1-p) of the Bernoulli reference
   const T* Jt;
+  const T* Jp;            // rows padded to ldp, as in PTParams
+  int ldp;
   const T* h;
   const T* temps;         // nt
   const double* rep_lr;   // chains*nt*n or null

@@ -189,34 +189,25 @@ struct Chain {
       for (int q = 0; q < 4; ++q) f[4 * g4 + q] += x.v[q] * dS;
     }
   }
-  // Add dS times a global row of length n.
-  __device__ __forceinline__ void add_row_global(const T* __restrict__ row, int n, bool vec, T dS) {
+  // Add dS times a coupling row padded with zeros to ldp elements (a
+  // multiple of 256, 16-byte aligned): a 256-element chunk is loaded whole
+  // or, beyond ldp, not at all (a wave-uniform test, no per-lane masks).
+  __device__ __forceinline__ void add_row_padded(const T* __restrict__ row, int ldp, T dS) {
     const int lane = threadIdx.x & 63;
-    if (vec) {  // n % 4 == 0: the row start is 16-byte aligned
-      const T4<T>* r4 = reinterpret_cast<const T4<T>*>(row);
-      T4<T> x[G / 4];
+    const T4<T>* r4 = reinterpret_cast<const T4<T>*>(row);
+    T4<T> x[G / 4];
+    if (ldp >= 64 * G) {
 #pragma unroll
-      for (int g4 = 0; g4 < G / 4; ++g4) {
-        const int i0 = (g4 << 8) + (lane << 2);
-        if (i0 < n)
-          x[g4] = r4[(g4 << 6) + lane];
-        else
-          x[g4] = T4<T>{{T(0), T(0), T(0), T(0)}};
-      }
+      for (int g4 = 0; g4 < G / 4; ++g4) x[g4] = r4[(g4 << 6) + lane];
+    } else {
 #pragma unroll
       for (int g4 = 0; g4 < G / 4; ++g4)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) f[4 * g4 + q] += x[g4].v[q] * dS;
-    } else {
-      T x[G];
-#pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const int i = elem_of(lane, g);
-        x[g] = (i < n) ? row[i] : T(0);
-      }
-#pragma unroll
-      for (int g = 0; g < G; ++g) f[g] += x[g] * dS;
+        x[g4] = ((g4 << 8) < ldp) ? r4[(g4 << 6) + lane] : T4<T>{{T(0), T(0), T(0), T(0)}};
     }
+#pragma unroll
+    for (int g4 = 0; g4 < G / 4; ++g4)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) f[4 * g4 + q] += x[g4].v[q] * dS;
   }
 };
 
@@ -265,11 +256,10 @@ struct SweepCtx {
 // from global memory (L2 / Infinity Cache resident) as 16-byte vectors.
 template <typename T, int G, bool AIS>
 __device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const SweepCtx& c,
-                               const T* __restrict__ Jt, T beta, uint64_t cid,
+                               const T* __restrict__ Jp, int ldp, T beta, uint64_t cid,
                                const double* rep_lr, const int* rep_perm,
                                unsigned long long& acc, int& od) {
   const int n = c.n, lane = threadIdx.x & 63;
-  const bool vec = (n & 3) == 0;
   od = 0;
   if (!active) return 0.0;
   FeistelKey fk;
@@ -289,20 +279,21 @@ __device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const 
         pos_l = (c.order == kSequential) ? vi : (int)feistel_perm(fk, (uint32_t)vi);
       lr_l = log_uniform(rep_lr, cid, c.sw, pos_l, c.k0, c.k1);
     }
+    const T h_l = (vi < n) ? hs[pos_l] : T(0);
     const int cnt = min(64, n - w);
     for (int t = 0; t < cnt; ++t) {
       const int pos = rl(pos_l, t);
       const double lr = rl(lr_l, t);
       const int pl = lane_of(pos), ps = slot_of(pos);
       const bool up = (rl(ch.bits, pl) >> ps) & 1;
-      const T fh = rl(ch.pick(ps), pl) + hs[pos];
+      const T fh = rl(ch.pick(ps), pl) + rl(h_l, t);
       double dn;
       if (decide<T, AIS>(c.ptype, up, fh, lr, (double)beta, c.ref, c.lratio, dn)) {
         sum += -dn;
         ++acc;
         od += up ? -1 : 1;
         if (lane == pl) ch.bits ^= (typename Chain<T, G>::B(1) << ps);
-        ch.add_row_global(Jt + (size_t)pos * n, n, vec,
+        ch.add_row_padded(Jp + (size_t)pos * ldp, ldp,
                           (c.ptype == 0) ? (up ? T(-2) : T(2)) : (up ? T(-1) : T(1)));
       }
     }
@@ -378,7 +369,7 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(P
       ch.load(p.fields + off, p.spins + off, n);
       const size_t rb = (((size_t)read * p.S + s) * R + r) * (size_t)n;
       int od;
-      const double d = sweep_direct<T, G, false>(ch, true, s_h, c, p.Jt, s_bs[r],
+      const double d = sweep_direct<T, G, false>(ch, true, s_h, c, p.Jp, p.ldp, s_bs[r],
                                                  (uint64_t)read * R + r,
                                                  p.rep_lr ? p.rep_lr + rb : nullptr,
                                                  p.rep_perm ? p.rep_perm + rb : nullptr, acc, od);
@@ -511,7 +502,7 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_ais_kernel(
     }
     const size_t rb = ((size_t)chain * p.nt + t) * (size_t)n;
     int od;
-    const double d = sweep_direct<T, G, true>(ch, true, s_h, c, p.Jt, beta, (uint64_t)chain,
+    const double d = sweep_direct<T, G, true>(ch, true, s_h, c, p.Jp, p.ldp, beta, (uint64_t)chain,
                                               p.rep_lr ? p.rep_lr + rb : nullptr,
                                               p.rep_perm ? p.rep_perm + rb : nullptr, acc, od);
     E += d;
