@@ -29,6 +29,12 @@ if _HERE not in sys.path:
     sys.path.insert(0, _HERE)
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch of the blocked kernel on the default workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script (FETCH_SIZE
+# scaled by 2.000 and WRITE_SIZE by 1.000, the factors a 1 GiB device-to-device
+# copy gives); summary in profiles/wishart_n1024_kernels.md.
+MEASURED_TRAFFIC = {("blocked", 512, 32, 5000): 32.17e9}
 BF16_PEAK_TFLOPS = 2500.0
 FP8_PEAK_TFLOPS = 5000.0
 
@@ -304,13 +310,16 @@ def cpu_baseline(target_s=12.0, threads=None):
         threads = min(16, len(os.sched_getaffinity(0)))
     oracle.load()
     oracle.metropolis(J, h, 0, betas, st, en, 5, order=0, seed=4, nthreads=threads)  # warm
-    t0 = time.perf_counter()
-    oracle.metropolis(J, h, 0, betas, st, en, 40, order=0, seed=5, nthreads=threads)
-    dt = time.perf_counter() - t0
-    sweeps = max(40, int(40 * target_s / max(dt, 1e-3)))
-    t0 = time.perf_counter()
-    oracle.metropolis(J, h, 0, betas, st, en, sweeps, order=0, seed=6, nthreads=threads)
-    dt = time.perf_counter() - t0
+    # Chunks of sweeps (continuing the same chain) until target_s has passed.
+    sweeps, dt, chunk = 0, 0.0, 40
+    while dt < target_s:
+        t0 = time.perf_counter()
+        out = oracle.metropolis(J, h, 0, betas, st, en, chunk, order=0, seed=6,
+                                sweep_offset=sweeps, nthreads=threads)
+        dt += time.perf_counter() - t0
+        sweeps += chunk
+        st, en = out["states"], out["energies"]
+        chunk = max(40, int(chunk * min(4.0, 0.5 * target_s / max(dt, 1e-3))))
     return {"value": n * R * sweeps / dt, "unit": "spin_updates/s", "cores": threads,
             "kind": "port",
             "sample": f"1 read x {R} replicas x {sweeps} sweeps of the same N=1024 Wishart "
@@ -367,7 +376,9 @@ def main():
         flop = wl.updates * 2 * n * 2  # split-bf16: two products per coupling
         achieved = flop / (H["kernel_ms_mean"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None}
+                "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
+                "traffic": MEASURED_TRAFFIC.get(("blocked", args.reads, args.replicas,
+                                                 args.sweeps))}
         dtype = "bf16x2/f32"
     line = {
         "metric": "spin updates/s + TTS(99%) at N=1024: MFMA-blocked vs plain kernel, same GPU",
