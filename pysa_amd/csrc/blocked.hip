@@ -38,6 +38,7 @@
 #include "blocked.hpp"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <type_traits>
@@ -668,6 +669,510 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
   }
 }
 
+// ------------------------------------------------------- pipelined form --
+// Four waves per group of 64 replicas.  Wave 0 runs the proposals with one
+// lane per replica: the lane holds all 32 fields of the block, so a decision
+// needs no cross-lane broadcast.  Waves 1-3 each own a third of the K range
+// and, while wave 0 sweeps block c, compute the next block's fields over
+// every K block except c (whose spins are changing):
+//     P_{c+1} = sum_{b != c} J[c+1, b] . S_b
+// When wave 0 is done with block c, the owner of block c takes the new spins
+// and adds the missing product J[c+1, c] . S_c (one K block on the matrix
+// cores, from fragments kept back while computing P_{c+1}); the fields of
+// block c+1 are P_{c+1} plus that correction.  The matrix products of one
+// block therefore run under the proposals of the previous one.  The matrix
+// waves also prepare the next block's acceptance brackets, so the chain only
+// compares, flips and updates.  Shared state alternates between two buffers
+// by the parity of a running block counter.
+constexpr int kPS = 36;  // row stride of a replica's 32 fields (16-byte rows)
+struct __align__(16) PipeLds {
+  float part[2][3][64 * kPS];  // [parity][matrix wave][replica * kPS + row]
+  float corr[64 * kPS];        // J[c+1, c] . S_c of the next block
+  float jb[2][kB * kB];       // J^T[pos0+i][pos0+k] at i*32 + k
+  float hb[2][kB];
+  float rsb[2][kB];
+  float lo[2][kB][64];        // acceptance brackets [parity][row][replica]
+  float hi[2][kB][64];
+  uint32_t bb[2][64];         // block spins before the block (bit r = row r)
+  uint32_t bnew[64];          // block spins after the block
+  float sfp[64 * 33];         // exact rerun staging
+  float E[64];
+  float rb[64];
+  int bi[64];
+  int flag[64];
+  int src[64];
+  double lu[64];
+};
+
+template <int NP, bool I8, int NW>
+__global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
+  __shared__ PipeLds L;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tid = threadIdx.x;
+  const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
+  const int read0 = blockIdx.x * p.rpw;
+  const int h = lane >> 5;
+  // Operand role (matrix waves): tile-0 replica qa and tile-1 replica qb.
+  const int qa = lane & 31, qb = 32 + (lane & 31);
+  const int ga = qa / R, gb = qb / R;
+  const bool acta = (ga < p.rpw) && (read0 + ga < p.reads);
+  const bool actb = (gb < p.rpw) && (read0 + gb < p.reads);
+  const size_t base_a = acta ? ((size_t)(read0 + ga) * R + (qa - ga * R)) * n : 0;
+  const size_t base_b = actb ? ((size_t)(read0 + gb) * R + (qb - gb * R)) * n : 0;
+  // Replica role (every wave): lane = replica q of the group.
+  const int q = lane;
+  const int g = q / R, rr = q - g * R;
+  const int read = read0 + g;
+  const bool act = (g < p.rpw) && (read < p.reads);
+  const uint64_t cid = (uint64_t)read * R + rr;
+  const bool mw = w > 0;  // a matrix wave
+  const int m = w - 1;
+  const int c0 = mw ? m * kb : 0;
+  const int jn = mw ? max(0, min(kb, nblk - c0)) : 0;
+  const int mt = tid - 64;  // matrix-lane index 0..191
+
+  Bits<NW> sa, sb;
+  sa.w = 0;
+  sb.w = 0;
+  for (int j = 0; j < jn; ++j) {
+    const int c = c0 + j;
+    uint32_t va = 0, vb = 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const int rel = I8 ? (16 * h + b) : ((b < 8) ? (8 * h + b) : (16 + 8 * h + (b - 8)));
+      const int pos = 32 * c + rel;
+      if (pos < n) {
+        if (acta && p.spins[base_a + pos] == 1) va |= 1u << b;
+        if (actb && p.spins[base_b + pos] == 1) vb |= 1u << b;
+      }
+    }
+    set_half<NW>(sa, j, va);
+    set_half<NW>(sb, j, vb);
+  }
+  float E = 0.f, bestE = 0.f, gE = 0.f;
+  int gsw = 0, gimp = 0;
+  unsigned long long acc = 0;
+  if (w == 0) {
+    E = act ? p.energies[(size_t)read * R + rr] : 0.f;
+    L.bi[q] = act ? rr : 0;  // beta_idx = arange(R) (sa.py:325)
+    L.rb[q] = act ? p.betas[rr] : 0.f;
+    bestE = E;
+    gE = E;  // simulation.py:205 (meaningful on read leaders, rr == 0)
+    if (p.best_sem == 0 && act) {
+      p.best_rep_e[(size_t)read * R + rr] = E;
+      p.best_rep_sw[(size_t)read * R + rr] = 0;
+    }
+  }
+  for (int i = tid; i < 64 * kPS; i += 256) L.corr[i] = 0.f;
+
+  constexpr int PF = 4;
+  const uint32_t emul = p.ptype == 0 ? 0xFEu : 1u;
+  const uint32_t exr = p.ptype == 0 ? 0xFFFFFFFFu : 0u;
+  constexpr int NST = I8 ? 1 : 2;
+  constexpr int NJB = (kB * kB + 191) / 192;
+  typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
+  Frag ring[PF][NST][NP], fcorr[NST][NP];
+  float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
+  auto frag_ptr = [&](int c, int j, int st, int pc) -> const Frag* {
+    if (I8) return reinterpret_cast<const Frag*>(p.frag) + ((size_t)c * nblk + c0 + j) * 64 + lane;
+    return reinterpret_cast<const Frag*>(p.frag) +
+           (((size_t)c * (2 * nblk) + 2 * (c0 + j) + st) * NP + pc) * 64 + lane;
+  };
+  auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
+#pragma unroll
+    for (int st = 0; st < NST; ++st)
+#pragma unroll
+      for (int pc = 0; pc < NP; ++pc) dst[st][pc] = *frag_ptr(c, j, st, pc);
+  };
+  // Request row block c: the first PF fragment stages of this wave's K
+  // range and (spread over the matrix lanes) J[blk][blk], h and row sums.
+  auto prefetch_block = [&](int c) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+      if (u < jn) load_stage(c, u, ring[u]);
+    const int pos0 = 32 * c;
+#pragma unroll
+    for (int e2 = 0; e2 < NJB; ++e2) {
+      const int e = mt + e2 * 192;
+      const int i = e >> 5, k = e & 31;
+      jb_reg[e2] = (e < kB * kB && pos0 + i < n && pos0 + k < n)
+                       ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
+    }
+    h_reg = (mt >= 0 && mt < kB && pos0 + mt < n) ? p.h[pos0 + mt] : 0.f;
+    rs_reg = (!I8 && p.rowsum && mt >= 0 && mt < kB && pos0 + mt < n) ? p.rowsum[pos0 + mt] : 0.f;
+  };
+  auto publish_block = [&](int pb) {
+#pragma unroll
+    for (int e2 = 0; e2 < NJB; ++e2) {
+      const int e = mt + e2 * 192;
+      if (e < kB * kB) L.jb[pb][e] = jb_reg[e2];
+    }
+    if (mt >= 0 && mt < kB) {
+      L.hb[pb][mt] = h_reg;
+      L.rsb[pb][mt] = rs_reg;
+    }
+  };
+  // Products of K step j (fragments fa) with spins (ha: tile 0, hb2: tile 1).
+  auto mfma_step = [&](const Frag (&fa)[NST][NP], uint32_t ha, uint32_t hb2, f32x16& acc0,
+                       f32x16& acc1, i32x16& iacc0, i32x16& iacc1) {
+    if (I8) {
+      uint32_t da[4], db[4];
+      spins_i8(ha, emul, exr, da);
+      spins_i8(hb2, emul, exr, db);
+      i32x4 ba, bbv;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        ba[e] = (int)da[e];
+        bbv[e] = (int)db[e];
+      }
+      const i32x4 a8 = __builtin_bit_cast(i32x4, fa[0][0]);
+      iacc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, ba, iacc0, 0, 0, 0);
+      iacc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, bbv, iacc1, 0, 0, 0);
+    } else {
+      const uint32_t nza = spins_nz(ha), nzb = spins_nz(hb2);
+#pragma unroll
+      for (int st = 0; st < NST; ++st) {
+        uint32_t da[4], db[4];
+        if (st == 0) {
+          spins_bf16<0>(nza, da);
+          spins_bf16<0>(nzb, db);
+        } else {
+          spins_bf16<1>(nza, da);
+          spins_bf16<1>(nzb, db);
+        }
+        u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
+        const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bbv = __builtin_bit_cast(bf16x8, ub);
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc) {
+          const bf16x8 a16 = __builtin_bit_cast(bf16x8, fa[st][pc]);
+          acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16, ba, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16, bbv, acc1, 0, 0, 0);
+        }
+      }
+    }
+  };
+  // Fields from the C layout (register v: row 8(v/4) + 4h + v%4 of column
+  // lane%32) into [replica][row], four consecutive rows per 16-byte store.
+  auto write_fields = [&](float* dst, const f32x16& acc0, const f32x16& acc1, const i32x16& iacc0,
+                          const i32x16& iacc1) {
+#pragma unroll
+    for (int g4 = 0; g4 < 4; ++g4) {
+      const int i = 8 * g4 + 4 * h;
+      float4 x, y;
+      if (I8) {
+        x = make_float4((float)iacc0[4 * g4], (float)iacc0[4 * g4 + 1], (float)iacc0[4 * g4 + 2],
+                        (float)iacc0[4 * g4 + 3]);
+        y = make_float4((float)iacc1[4 * g4], (float)iacc1[4 * g4 + 1], (float)iacc1[4 * g4 + 2],
+                        (float)iacc1[4 * g4 + 3]);
+      } else {
+        x = make_float4(acc0[4 * g4], acc0[4 * g4 + 1], acc0[4 * g4 + 2], acc0[4 * g4 + 3]);
+        y = make_float4(acc1[4 * g4], acc1[4 * g4 + 1], acc1[4 * g4 + 2], acc1[4 * g4 + 3]);
+      }
+      *reinterpret_cast<float4*>(&dst[(lane & 31) * kPS + i]) = x;
+      *reinterpret_cast<float4*>(&dst[(32 + (lane & 31)) * kPS + i]) = y;
+    }
+  };
+  // Partial fields of row block cr over this wave's K range, skipping K block
+  // ex (its fragments are kept in fcorr for the correction); ring holds the
+  // first PF stages of row block cr on entry.
+  auto compute_part = [&](int cr, int ex, float* dst) {
+    f32x16 acc0 = {}, acc1 = {};
+    i32x16 iacc0 = {}, iacc1 = {};
+    constexpr int kUnrollJ = NW <= 8 ? (2 * NW + PF - 1) / PF : 1;
+#pragma unroll kUnrollJ
+    for (int jb0 = 0; jb0 < 2 * NW; jb0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int j = jb0 + u;
+        if (j >= jn) continue;
+        Frag fa[NST][NP];
+#pragma unroll
+        for (int st = 0; st < NST; ++st)
+#pragma unroll
+          for (int pc = 0; pc < NP; ++pc) fa[st][pc] = ring[u][st][pc];
+        if (j + PF < jn) load_stage(cr, j + PF, ring[u]);
+        if (c0 + j == ex) {
+#pragma unroll
+          for (int st = 0; st < NST; ++st)
+#pragma unroll
+            for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = fa[st][pc];
+          continue;
+        }
+        mfma_step(fa, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
+      }
+    }
+    write_fields(dst, acc0, acc1, iacc0, iacc1);
+  };
+  // Acceptance brackets of row block cr for sweep counter sw: this matrix
+  // wave's share of the rows, lane = replica (beta from L.rb).
+  auto brackets = [&](int cr, uint64_t sw, int pb) {
+    const float beta = L.rb[q];
+    const float inv_b = (beta > 0.f) ? 1.0f / beta : 0.f;
+    const int pos0 = 32 * cr;
+    const int rlo = m * 12, rhi = min(kB, rlo + 12);  // whole Philox blocks of 4 rows
+    u32x4 blk = {0u, 0u, 0u, 0u};
+    for (int r = rlo; r < rhi; ++r) {
+      const int pos = pos0 + r;
+      if ((pos & 3) == 0 || r == rlo) blk = sweep_block(cid, sw, (uint32_t)pos, p.k0, p.k1);
+      float lo, hi;
+      accept_bounds(pick_word(blk, (uint32_t)pos & 3u), beta, inv_b, lo, hi);
+      if (pos >= n || !act) lo = hi = INFINITY;  // padding and idle lanes: never taken
+      L.lo[pb][r][q] = lo;
+      L.hi[pb][r][q] = hi;
+    }
+  };
+  auto owner_of = [&](int c) { return 1 + c / kb; };  // wave owning K block c
+  auto publish_bits = [&](int c, int pb) {  // by the owner of block c
+    const int j = c - c0;
+    const uint32_t mine = (lane < 32) ? get_half<NW>(sa, j) : get_half<NW>(sb, j);
+    const uint32_t give = (lane < 32) ? get_half<NW>(sb, j) : get_half<NW>(sa, j);
+    const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
+    L.bb[pb][lane] = (lane < 32) ? block_of<I8>(mine, got) : block_of<I8>(got, mine);
+  };
+
+  // ---- prologue: the fields of block 0 in full, its brackets and spins
+  const uint64_t sw0 = p.sweep_offset;
+  if (mw) {
+    prefetch_block(0);
+    publish_block(0);
+    compute_part(0, -1, L.part[0][m]);
+    prefetch_block(nblk > 1 ? 1 : 0);
+    brackets(0, sw0, 0);
+    if (w == owner_of(0)) publish_bits(0, 0);
+  }
+  __syncthreads();
+
+  const float cu = (p.ptype == 0) ? 2.f : 1.f;  // |dn / (F + h)|
+  uint32_t t = 0;                               // running block counter
+  for (int s = 0; s < p.S; ++s) {
+    const uint64_t sw = (uint64_t)s + p.sweep_offset;
+    const float beta = act ? L.rb[q] : 0.f;
+    double dsum = 0.0;
+    uint32_t nacc = 0;
+    for (int c = 0; c < nblk; ++c, ++t) {
+      const int pb = t & 1, pn = pb ^ 1;
+      const int c1 = (c + 1 == nblk) ? 0 : c + 1;
+      const int pos0 = 32 * c;
+      if (w == 0) {
+        // ---- proposals of block c, one lane per replica
+        float fp[kB];
+        {
+          const float4* p0 = reinterpret_cast<const float4*>(&L.part[pb][0][q * kPS]);
+          const float4* p1 = reinterpret_cast<const float4*>(&L.part[pb][1][q * kPS]);
+          const float4* p2 = reinterpret_cast<const float4*>(&L.part[pb][2][q * kPS]);
+          const float4* pc = reinterpret_cast<const float4*>(&L.corr[q * kPS]);
+#pragma unroll
+          for (int r4 = 0; r4 < kB / 4; ++r4) {
+            const float4 a0 = p0[r4], a1 = p1[r4], a2 = p2[r4], a3 = pc[r4];
+            const float va[4] = {a0.x, a0.y, a0.z, a0.w}, vb[4] = {a1.x, a1.y, a1.z, a1.w};
+            const float vc[4] = {a2.x, a2.y, a2.z, a2.w}, vd[4] = {a3.x, a3.y, a3.z, a3.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int r = 4 * r4 + e;
+              float v = va[e] + vb[e];
+              v = v + vc[e];
+              v = v + vd[e];
+              // QUBO with bf16 operands: the product used +-1 spins, F = (J.s + J.1) / 2
+              if (!I8 && p.ptype == 1) v = 0.5f * (v + L.rsb[pb][r]);
+              fp[r] = v + L.hb[pb][r];
+            }
+          }
+        }
+        uint32_t mb = L.bb[pb][q];
+        const uint32_t mb0 = mb;
+        float dpart = 0.f;
+        bool amb = false;
+        // Rows of J[blk][blk] (the same for every lane) stream from LDS
+        // through a ring JR rows ahead of the chain.
+        constexpr int JR = 3;
+        const float4* jb4 = reinterpret_cast<const float4*>(L.jb[pb]);
+        float4 jring[JR][kB / 4];
+#pragma unroll
+        for (int d = 0; d < JR; ++d)
+#pragma unroll
+          for (int r4 = 0; r4 < kB / 4; ++r4) jring[d][r4] = jb4[d * (kB / 4) + r4];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+          const bool up = (mb >> k) & 1u;
+          // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
+          const float sg = up ? cu : -cu;
+          const float dn = fp[k] * sg;
+          const float hik = L.hi[pb][k][q], lok = L.lo[pb][k][q];
+          const bool take = act & (dn > hik);
+          amb |= act & !take & !(dn < lok);
+          mb ^= (uint32_t)take << k;
+          dpart += take ? -dn : 0.f;
+          const float dS = take ? -sg : 0.f;
+#pragma unroll
+          for (int r4 = 0; r4 < kB / 4; ++r4) {
+            const float4 j4 = jring[k % JR][r4];
+            // dS is 0, +-1 or +-2, so the fused form rounds like mul + add
+            fp[4 * r4] = __builtin_fmaf(j4.x, dS, fp[4 * r4]);
+            fp[4 * r4 + 1] = __builtin_fmaf(j4.y, dS, fp[4 * r4 + 1]);
+            fp[4 * r4 + 2] = __builtin_fmaf(j4.z, dS, fp[4 * r4 + 2]);
+            fp[4 * r4 + 3] = __builtin_fmaf(j4.w, dS, fp[4 * r4 + 3]);
+          }
+          if (k + JR < kB)
+#pragma unroll
+            for (int r4 = 0; r4 < kB / 4; ++r4) jring[k % JR][r4] = jb4[(k + JR) * (kB / 4) + r4];
+        }
+        if (__builtin_expect(__any(amb), 0)) {
+          // A decision fell inside its bracket: rerun the block with the
+          // exact test on every proposal (rolled loop, fields staged in LDS).
+          float* sf = &L.sfp[q * 33];
+          for (int r = 0; r < kB; ++r) {
+            float v = L.part[pb][0][q * kPS + r] + L.part[pb][1][q * kPS + r];
+            v = v + L.part[pb][2][q * kPS + r];
+            v = v + L.corr[q * kPS + r];
+            if (!I8 && p.ptype == 1) v = 0.5f * (v + L.rsb[pb][r]);
+            sf[r] = v + L.hb[pb][r];
+          }
+          mb = mb0;
+          dpart = 0.f;
+          for (int k = 0; k < kB; ++k) {
+            const bool up = (mb >> k) & 1u;
+            const float sg = up ? cu : -cu;
+            const float dn = sf[k] * sg;
+            bool take = false;
+            if (act && pos0 + k < n) {
+              const uint32_t wd = pick_word(sweep_block(cid, sw, (uint32_t)(pos0 + k), p.k0, p.k1),
+                                            (uint32_t)(pos0 + k) & 3u);
+              take = accept_exact(wd, beta, dn);
+            }
+            mb ^= (uint32_t)take << k;
+            dpart += take ? -dn : 0.f;
+            const float dS = take ? -sg : 0.f;
+            for (int r = 0; r < kB; ++r) sf[r] = __builtin_fmaf(L.jb[pb][k * kB + r], dS, sf[r]);
+          }
+        }
+        L.bnew[q] = mb;
+        dsum += (double)dpart;
+        nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
+      } else {
+        // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
+        publish_block(pn);
+        compute_part(c1, c, L.part[pn][m]);
+        prefetch_block(c1 + 1 < nblk ? c1 + 1 : 0);
+        if (c1 != 0) brackets(c1, sw, pn);
+      }
+      __syncthreads();
+      if (mw && w == owner_of(c)) {
+        // ---- the owner of block c takes its new spins and adds J[c1, c] . S_c
+        const int j = c - c0;
+        set_half<NW>(sa, j, half_of<I8>(L.bnew[qa], h));
+        set_half<NW>(sb, j, half_of<I8>(L.bnew[qb], h));
+        f32x16 acc0 = {}, acc1 = {};
+        i32x16 iacc0 = {}, iacc1 = {};
+        mfma_step(fcorr, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
+        write_fields(L.corr, acc0, acc1, iacc0, iacc1);
+      }
+      if (mw && w == owner_of(c1)) publish_bits(c1, pn);
+      if (c + 1 == nblk) {
+        // ---- end of sweep: energy (simulation.py:157-158), best states, PT
+        if (w == 0) {
+          acc += nacc;
+          E = (float)((double)E + dsum);
+          L.E[q] = E;
+          if (p.best_sem == 0) {  // simulation.py:161-164
+            const bool imp = act && (E < bestE);
+            if (imp) {
+              bestE = E;
+              p.best_rep_e[(size_t)read * R + rr] = E;
+              p.best_rep_sw[(size_t)read * R + rr] = s;
+            }
+            L.flag[q] = imp ? 1 : 0;
+          }
+          if (p.use_pt && act && rr < R - 1)
+            L.lu[q] = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)rr, p.k0, p.k1),
+                                        (uint32_t)rr & 3u));
+        }
+        __syncthreads();
+        if (mw && p.best_sem == 0) {
+          if (acta && L.flag[qa]) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_a);
+          if (actb && L.flag[qb]) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_rep + base_b);
+        }
+        if (w == 0 && act && rr == 0) {  // read leader
+          const int b0 = g * R;
+          int src = -1;
+          if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
+            float cur = gE;
+            for (int k = 0; k < R; ++k)
+              if (L.E[b0 + k] < cur) {
+                cur = L.E[b0 + k];
+                src = k;
+              }
+            if (src >= 0) {
+              gE = cur;
+              gsw = s;
+              gimp = 1;
+            }
+          }
+          if (p.use_pt) {  // utils.py:44-58
+            for (int k = 0; k < R - 1; ++k) {
+              const int a = R - k - 1, b = R - k - 2;
+              const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
+                               (p.betas[a] - p.betas[b]);
+              if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
+                const int tt = L.bi[b0 + a];
+                L.bi[b0 + a] = L.bi[b0 + b];
+                L.bi[b0 + b] = tt;
+              }
+            }
+          }
+          for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = p.betas[k];
+          if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
+            double lw = 0.0, bold = 0.0;
+            for (int r = 1; r < R; ++r) {
+              const int k = p.bsort[r];
+              const float be = p.betas[k];
+              const float en = L.E[b0 + L.bi[b0 + k]];
+              const float a = -be * en;
+              lw += (double)a - (-bold * (double)en);
+              bold = (double)be;
+            }
+            p.log_omegas[(size_t)read * p.S + s] = lw;
+          }
+          L.src[g] = src;
+        }
+        __syncthreads();
+        if (mw && p.best_sem == 1) {  // owners copy their part of a new global best
+          if (acta && L.src[ga] == qa - ga * R)
+            store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
+          if (actb && L.src[gb] == qb - gb * R)
+            store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
+        }
+        if (mw) brackets(0, sw + 1, pn);  // next sweep's first block, with the new betas
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- outputs
+  if (mw) {
+    if (acta) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.spins + base_a);
+    if (actb) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.spins + base_b);
+  }
+  if (w == 0 && act) {
+    if (p.accepted) atomicAdd(p.accepted + read, acc);
+    p.energies[(size_t)read * R + rr] = E;
+    p.beta_idx[(size_t)read * R + rr] = L.bi[q];
+    if (rr == 0 && p.best_sem == 1) {
+      p.best_energy[read] = gE;
+      p.best_sweeps[read] = gsw;
+      L.src[g] = gimp;
+    }
+  }
+  if (p.best_sem == 1) {
+    // never improved: best_state is still the view of states[0] (simulation.py:204)
+    __syncthreads();
+    if (mw) {
+      if (acta && qa - ga * R == 0 && !L.src[ga])
+        store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
+      if (actb && qb - gb * R == 0 && !L.src[gb])
+        store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
+    }
+  }
+}
+
 // Final argmin over per-replica bests (simulation.py:174-177), one thread per read.
 __global__ void best_argmin_kernel(BParams p) {
   const int read = blockIdx.x * blockDim.x + threadIdx.x;
@@ -690,6 +1195,18 @@ hipError_t launch_kw(const BParams& p, int nw, int groups, hipStream_t st) {
     case 8: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 8, KW>), grid, block, 0, st, p); break;
     case 16: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 16, KW>), grid, block, 0, st, p); break;
     case 32: hipLaunchKernelGGL((blocked_pt_kernel<NP, I8, 32, KW>), grid, block, 0, st, p); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int NP, bool I8>
+hipError_t launch_pipe(const BParams& p, int nw, int groups, hipStream_t st) {
+  dim3 grid(groups), block(256);
+  switch (nw) {
+    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4>), grid, block, 0, st, p); break;
+    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8>), grid, block, 0, st, p); break;
+    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16>), grid, block, 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -756,6 +1273,17 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   const int groups = (reads + (64 / R) - 1) / (64 / R);
   int kw = 1;
   while (kw < 4 && 2 * kw <= nblk && groups * kw < 1024) kw *= 2;
+  // The pipelined kernel (proposal wave + three matrix waves) takes every
+  // problem of at least three blocks; PYSA_AMD_BLOCKED_SPLITK=1 selects the
+  // split-K kernel instead (kept for small problems and comparisons).
+  static const bool force_splitk = [] {
+    const char* e = getenv("PYSA_AMD_BLOCKED_SPLITK");
+    return e && e[0] == '1';
+  }();
+  const bool pipe = nblk >= 3 && !force_splitk;
+  const int kbp = (nblk + 2) / 3;
+  int nwp = 4;
+  while (2 * nwp < kbp) nwp *= 2;
   const int kb = (nblk + kw - 1) / kw;
   int nw = 4;
   while (2 * nw < kb) nw *= 2;  // words of 2 blocks each: kb <= 2 * nw
@@ -837,7 +1365,7 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   p.use_pt = use_pt;
   p.logw = logw;
   p.best_sem = best_sem;
-  p.kb = kb;
+  p.kb = pipe ? kbp : kb;
   p.k0 = (uint32_t)seed;
   p.k1 = (uint32_t)(seed >> 32);
   p.sweep_offset = sweep_offset;
@@ -874,7 +1402,16 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   }
   (void)hipEventRecord(e0, st);
   hipError_t le;
-  if (i8)
+  if (pipe) {
+    if (i8)
+      le = launch_pipe<1, true>(p, nwp, groups, st);
+    else if (np == 1)
+      le = launch_pipe<1, false>(p, nwp, groups, st);
+    else if (np == 2)
+      le = launch_pipe<2, false>(p, nwp, groups, st);
+    else
+      le = launch_pipe<3, false>(p, nwp, groups, st);
+  } else if (i8)
     le = launch_np<1, true>(p, nw, kw, groups, st);
   else if (np == 1)
     le = launch_np<1, false>(p, nw, kw, groups, st);
