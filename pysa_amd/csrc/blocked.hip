@@ -684,10 +684,14 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
 // waves also prepare the next block's acceptance brackets, so the chain only
 // compares, flips and updates.  Shared state alternates between two buffers
 // by the parity of a running block counter.
-constexpr int kPS = 36;  // row stride of a replica's 32 fields (16-byte rows)
+// A replica's 32 fields take 32 floats; its 16-byte row groups are XOR-
+// swizzled by the replica index so that 8 lanes of a 16-byte access hit all
+// 32 banks.
+__device__ __forceinline__ int fld(int q, int r) { return q * 32 + 4 * ((r >> 2) ^ (q & 7)) + (r & 3); }
+template <int MW>
 struct __align__(16) PipeLds {
-  float part[2][3][64 * kPS];  // [parity][matrix wave][replica * kPS + row]
-  float corr[64 * kPS];        // J[c+1, c] . S_c of the next block
+  float part[2][MW][64 * 32];  // [parity][matrix wave][fld(replica, row)]
+  float corr[64 * 32];         // J[c+1, c] . S_c of the next block
   float jb[2][kB * kB];       // J^T[pos0+i][pos0+k] at i*32 + k
   float hb[2][kB];
   float rsb[2][kB];
@@ -704,9 +708,9 @@ struct __align__(16) PipeLds {
   double lu[64];
 };
 
-template <int NP, bool I8, int NW>
-__global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
-  __shared__ PipeLds L;
+template <int NP, bool I8, int NW, int MW>
+__global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) {
+  __shared__ PipeLds<MW> L;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tid = threadIdx.x;
   const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
   const int read0 = blockIdx.x * p.rpw;
@@ -728,7 +732,7 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
   const int m = w - 1;
   const int c0 = mw ? m * kb : 0;
   const int jn = mw ? max(0, min(kb, nblk - c0)) : 0;
-  const int mt = tid - 64;  // matrix-lane index 0..191
+  const int mt = tid - 64;  // matrix-lane index 0..64*MW-1
 
   Bits<NW> sa, sb;
   sa.w = 0;
@@ -762,13 +766,13 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
       p.best_rep_sw[(size_t)read * R + rr] = 0;
     }
   }
-  for (int i = tid; i < 64 * kPS; i += 256) L.corr[i] = 0.f;
+  for (int i = tid; i < 64 * 32; i += 64 * (MW + 1)) L.corr[i] = 0.f;
 
   constexpr int PF = 4;
   const uint32_t emul = p.ptype == 0 ? 0xFEu : 1u;
   const uint32_t exr = p.ptype == 0 ? 0xFFFFFFFFu : 0u;
   constexpr int NST = I8 ? 1 : 2;
-  constexpr int NJB = (kB * kB + 191) / 192;
+  constexpr int NJB = (kB * kB + 64 * MW - 1) / (64 * MW);
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
   Frag ring[PF][NST][NP], fcorr[NST][NP];
   float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
@@ -792,7 +796,7 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
     const int pos0 = 32 * c;
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = mt + e2 * 192;
+      const int e = mt + e2 * 64 * MW;
       const int i = e >> 5, k = e & 31;
       jb_reg[e2] = (e < kB * kB && pos0 + i < n && pos0 + k < n)
                        ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
@@ -803,7 +807,7 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
   auto publish_block = [&](int pb) {
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = mt + e2 * 192;
+      const int e = mt + e2 * 64 * MW;
       if (e < kB * kB) L.jb[pb][e] = jb_reg[e2];
     }
     if (mt >= 0 && mt < kB) {
@@ -867,8 +871,8 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
         x = make_float4(acc0[4 * g4], acc0[4 * g4 + 1], acc0[4 * g4 + 2], acc0[4 * g4 + 3]);
         y = make_float4(acc1[4 * g4], acc1[4 * g4 + 1], acc1[4 * g4 + 2], acc1[4 * g4 + 3]);
       }
-      *reinterpret_cast<float4*>(&dst[(lane & 31) * kPS + i]) = x;
-      *reinterpret_cast<float4*>(&dst[(32 + (lane & 31)) * kPS + i]) = y;
+      *reinterpret_cast<float4*>(&dst[fld(lane & 31, i)]) = x;
+      *reinterpret_cast<float4*>(&dst[fld(32 + (lane & 31), i)]) = y;
     }
   };
   // Partial fields of row block cr over this wave's K range, skipping K block
@@ -908,16 +912,19 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
     const float beta = L.rb[q];
     const float inv_b = (beta > 0.f) ? 1.0f / beta : 0.f;
     const int pos0 = 32 * cr;
-    const int rlo = m * 12, rhi = min(kB, rlo + 12);  // whole Philox blocks of 4 rows
-    u32x4 blk = {0u, 0u, 0u, 0u};
-    for (int r = rlo; r < rhi; ++r) {
-      const int pos = pos0 + r;
-      if ((pos & 3) == 0 || r == rlo) blk = sweep_block(cid, sw, (uint32_t)pos, p.k0, p.k1);
-      float lo, hi;
-      accept_bounds(pick_word(blk, (uint32_t)pos & 3u), beta, inv_b, lo, hi);
-      if (pos >= n || !act) lo = hi = INFINITY;  // padding and idle lanes: never taken
-      L.lo[pb][r][q] = lo;
-      L.hi[pb][r][q] = hi;
+    // groups of 4 rows (one Philox block each) g = m, m + MW, ...
+    for (int g4 = m; g4 < kB / 4; g4 += MW) {
+      const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
+      const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = 4 * g4 + e, pos = pos0 + r;
+        float lo, hi;
+        accept_bounds(wd[e], beta, inv_b, lo, hi);
+        if (pos >= n || !act) lo = hi = INFINITY;  // padding and idle lanes: never taken
+        L.lo[pb][r][q] = lo;
+        L.hi[pb][r][q] = hi;
+      }
     }
   };
   auto owner_of = [&](int c) { return 1 + c / kb; };  // wave owning K block c
@@ -956,21 +963,24 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
         // ---- proposals of block c, one lane per replica
         float fp[kB];
         {
-          const float4* p0 = reinterpret_cast<const float4*>(&L.part[pb][0][q * kPS]);
-          const float4* p1 = reinterpret_cast<const float4*>(&L.part[pb][1][q * kPS]);
-          const float4* p2 = reinterpret_cast<const float4*>(&L.part[pb][2][q * kPS]);
-          const float4* pc = reinterpret_cast<const float4*>(&L.corr[q * kPS]);
 #pragma unroll
           for (int r4 = 0; r4 < kB / 4; ++r4) {
-            const float4 a0 = p0[r4], a1 = p1[r4], a2 = p2[r4], a3 = pc[r4];
-            const float va[4] = {a0.x, a0.y, a0.z, a0.w}, vb[4] = {a1.x, a1.y, a1.z, a1.w};
-            const float vc[4] = {a2.x, a2.y, a2.z, a2.w}, vd[4] = {a3.x, a3.y, a3.z, a3.w};
+            const int o = fld(q, 4 * r4);
+            float4 sm = *reinterpret_cast<const float4*>(&L.part[pb][0][o]);
+#pragma unroll
+            for (int mm = 1; mm < MW; ++mm) {
+              const float4 a = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
+              sm.x = sm.x + a.x;
+              sm.y = sm.y + a.y;
+              sm.z = sm.z + a.z;
+              sm.w = sm.w + a.w;
+            }
+            const float4 cr = *reinterpret_cast<const float4*>(&L.corr[o]);
+            const float vs[4] = {sm.x + cr.x, sm.y + cr.y, sm.z + cr.z, sm.w + cr.w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
               const int r = 4 * r4 + e;
-              float v = va[e] + vb[e];
-              v = v + vc[e];
-              v = v + vd[e];
+              float v = vs[e];
               // QUBO with bf16 operands: the product used +-1 spins, F = (J.s + J.1) / 2
               if (!I8 && p.ptype == 1) v = 0.5f * (v + L.rsb[pb][r]);
               fp[r] = v + L.hb[pb][r];
@@ -1020,9 +1030,9 @@ __global__ __launch_bounds__(256) void blocked_pipe_kernel(BParams p) {
           // exact test on every proposal (rolled loop, fields staged in LDS).
           float* sf = &L.sfp[q * 33];
           for (int r = 0; r < kB; ++r) {
-            float v = L.part[pb][0][q * kPS + r] + L.part[pb][1][q * kPS + r];
-            v = v + L.part[pb][2][q * kPS + r];
-            v = v + L.corr[q * kPS + r];
+            float v = L.part[pb][0][fld(q, r)];
+            for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(q, r)];
+            v = v + L.corr[fld(q, r)];
             if (!I8 && p.ptype == 1) v = 0.5f * (v + L.rsb[pb][r]);
             sf[r] = v + L.hb[pb][r];
           }
@@ -1200,13 +1210,18 @@ hipError_t launch_kw(const BParams& p, int nw, int groups, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Matrix waves of the pipelined kernel.  Four waves per workgroup keep one
+// wave per SIMD and its 512-VGPR budget; more waves would cap each at 256
+// registers, which the proposal wave's 32 fields and brackets exceed.
+constexpr int kPipeMW = 3;
+
 template <int NP, bool I8>
 hipError_t launch_pipe(const BParams& p, int nw, int groups, hipStream_t st) {
-  dim3 grid(groups), block(256);
+  dim3 grid(groups), block(64 * (kPipeMW + 1));
   switch (nw) {
-    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4>), grid, block, 0, st, p); break;
-    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8>), grid, block, 0, st, p); break;
-    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16>), grid, block, 0, st, p); break;
+    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW>), grid, block, 0, st, p); break;
+    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW>), grid, block, 0, st, p); break;
+    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW>), grid, block, 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1280,8 +1295,8 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     const char* e = getenv("PYSA_AMD_BLOCKED_SPLITK");
     return e && e[0] == '1';
   }();
-  const bool pipe = nblk >= 3 && !force_splitk;
-  const int kbp = (nblk + 2) / 3;
+  const bool pipe = nblk >= kPipeMW && !force_splitk;
+  const int kbp = (nblk + kPipeMW - 1) / kPipeMW;
   int nwp = 4;
   while (2 * nwp < kbp) nwp *= 2;
   const int kb = (nblk + kw - 1) / kw;
