@@ -711,7 +711,7 @@ struct __align__(16) PipeLds {
 template <int NP, bool I8, int NW, int MW>
 __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) {
   __shared__ PipeLds<MW> L;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, tid = threadIdx.x;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tid = threadIdx.x;
   const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
   const int read0 = blockIdx.x * p.rpw;
   const int h = lane >> 5;
@@ -781,18 +781,21 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     return reinterpret_cast<const Frag*>(p.frag) +
            (((size_t)c * (2 * nblk) + 2 * (c0 + j) + st) * NP + pc) * 64 + lane;
   };
+  // Stage j of this wave's K range; stages past the range reload the last
+  // K block, so every path issues the same loads and the wait counts stay
+  // exact (a skipped load would make the compiler wait for all of them).
   auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
+    const int jc = min(c0 + j, nblk - 1) - c0;
 #pragma unroll
     for (int st = 0; st < NST; ++st)
 #pragma unroll
-      for (int pc = 0; pc < NP; ++pc) dst[st][pc] = *frag_ptr(c, j, st, pc);
+      for (int pc = 0; pc < NP; ++pc) dst[st][pc] = *frag_ptr(c, jc, st, pc);
   };
   // Request row block c: the first PF fragment stages of this wave's K
   // range and (spread over the matrix lanes) J[blk][blk], h and row sums.
   auto prefetch_block = [&](int c) {
 #pragma unroll
-    for (int u = 0; u < PF; ++u)
-      if (u < jn) load_stage(c, u, ring[u]);
+    for (int u = 0; u < PF; ++u) load_stage(c, u, ring[u]);
     const int pos0 = 32 * c;
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
@@ -884,24 +887,26 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     constexpr int kUnrollJ = NW <= 8 ? (2 * NW + PF - 1) / PF : 1;
 #pragma unroll kUnrollJ
     for (int jb0 = 0; jb0 < 2 * NW; jb0 += PF) {
+      if (jb0 >= jn) break;
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int j = jb0 + u;
-        if (j >= jn) continue;
         Frag fa[NST][NP];
 #pragma unroll
         for (int st = 0; st < NST; ++st)
 #pragma unroll
           for (int pc = 0; pc < NP; ++pc) fa[st][pc] = ring[u][st][pc];
-        if (j + PF < jn) load_stage(cr, j + PF, ring[u]);
-        if (c0 + j == ex) {
+        load_stage(cr, j + PF, ring[u]);
+        if (j < jn) {
+          if (c0 + j == ex) {
 #pragma unroll
-          for (int st = 0; st < NST; ++st)
+            for (int st = 0; st < NST; ++st)
 #pragma unroll
-            for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = fa[st][pc];
-          continue;
+              for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = fa[st][pc];
+          } else {
+            mfma_step(fa, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
+          }
         }
-        mfma_step(fa, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
       }
     }
     write_fields(dst, acc0, acc1, iacc0, iacc1);
