@@ -781,11 +781,12 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     return reinterpret_cast<const Frag*>(p.frag) +
            (((size_t)c * (2 * nblk) + 2 * (c0 + j) + st) * NP + pc) * 64 + lane;
   };
-  // Stage j of this wave's K range; stages past the range reload the last
-  // K block, so every path issues the same loads and the wait counts stay
-  // exact (a skipped load would make the compiler wait for all of them).
+  // Stage j of this wave's K range; stages past the range reload the
+  // range's last K block (recently read, so a cache hit), so every path
+  // issues the same loads and the wait counts stay exact (a skipped load
+  // would make the compiler wait for all of them).
   auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
-    const int jc = min(c0 + j, nblk - 1) - c0;
+    const int jc = jn > 0 ? min(j, jn - 1) : nblk - 1 - c0;
 #pragma unroll
     for (int st = 0; st < NST; ++st)
 #pragma unroll
@@ -1031,8 +1032,9 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
             for (int r4 = 0; r4 < kB / 4; ++r4) jring[k % JR][r4] = jb4[(k + JR) * (kB / 4) + r4];
         }
         if (__builtin_expect(__any(amb), 0)) {
-          // A decision fell inside its bracket: rerun the block with the
-          // exact test on every proposal (rolled loop, fields staged in LDS).
+          // A decision fell inside its bracket: rerun the block, taking the
+          // exact test for the decisions inside their brackets (rolled loop,
+          // fields staged in LDS).
           float* sf = &L.sfp[q * 33];
           for (int r = 0; r < kB; ++r) {
             float v = L.part[pb][0][fld(q, r)];
@@ -1047,8 +1049,9 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
             const bool up = (mb >> k) & 1u;
             const float sg = up ? cu : -cu;
             const float dn = sf[k] * sg;
-            bool take = false;
-            if (act && pos0 + k < n) {
+            // idle lanes and padding rows have lo = hi = +inf: never taken
+            bool take = act & (dn > L.hi[pb][k][q]);
+            if (act && !take && !(dn < L.lo[pb][k][q])) {
               const uint32_t wd = pick_word(sweep_block(cid, sw, (uint32_t)(pos0 + k), p.k0, p.k1),
                                             (uint32_t)(pos0 + k) & 3u);
               take = accept_exact(wd, beta, dn);
@@ -1065,8 +1068,9 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
       } else {
         // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
         publish_block(pn);
+        const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
         compute_part(c1, c, L.part[pn][m]);
-        prefetch_block(c1 + 1 < nblk ? c1 + 1 : 0);
+        prefetch_block(c2);
         if (c1 != 0) brackets(c1, sw, pn);
       }
       __syncthreads();
