@@ -153,6 +153,14 @@ __device__ __forceinline__ void spins_bf16(uint32_t nz, uint32_t (&d)[4]) {
   for (int e = 0; e < 4; ++e) d[e] = ((nz << (15 - 8 * ST - 2 * e)) & 0x80008000u) | 0x3F803F80u;
 }
 __device__ __forceinline__ uint32_t spins_nz(uint32_t y) { return ~(y | (y << 15)); }
+// The same with the mask and the +-1 pattern held in registers (vm =
+// 0x80008000, vc = 0x3F803F80), so each word is a shift and one AND-OR.
+template <int ST>
+__device__ __forceinline__ void spins_bf16_r(uint32_t nz, uint32_t vm, uint32_t vc,
+                                             uint32_t (&d)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) d[e] = ((nz << (15 - 8 * ST - 2 * e)) & vm) | vc;
+}
 // Sixteen spins (bits of x) as int8: ising (mul, xr) = (0xFE, 0xFFFFFFFF)
 // gives +1 / -1 per byte, qubo (1, 0) gives 1 / 0.
 __device__ __forceinline__ void spins_i8(uint32_t x, uint32_t mul, uint32_t xr, uint32_t (&d)[4]) {
@@ -684,6 +692,12 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
 // waves also prepare the next block's acceptance brackets, so the chain only
 // compares, flips and updates.  Shared state alternates between two buffers
 // by the parity of a running block counter.
+// Matrix waves of the pipelined kernel.  Four waves per workgroup keep one
+// wave per SIMD and its 512-VGPR budget; more waves would cap each at 256
+// registers, which the proposal wave's 32 fields and brackets exceed.
+constexpr int kPipeMW = 3;
+constexpr int kPipePF = 4;  // fragment stages in flight per matrix wave (slot ring)
+
 // A replica's 32 fields take 32 floats; its 16-byte row groups are XOR-
 // swizzled by the replica index so that 8 lanes of a 16-byte access hit all
 // 32 banks.
@@ -768,9 +782,12 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   }
   for (int i = tid; i < 64 * 32; i += 64 * (MW + 1)) L.corr[i] = 0.f;
 
-  constexpr int PF = 4;
+  constexpr int PF = kPipePF;
   const uint32_t emul = p.ptype == 0 ? 0xFEu : 1u;
   const uint32_t exr = p.ptype == 0 ? 0xFFFFFFFFu : 0u;
+  // operand-expansion constants in vector registers (see spins_bf16_r)
+  uint32_t vmask = 0x80008000u, vone = 0x3F803F80u;
+  asm volatile("" : "+v"(vmask), "+v"(vone));
   constexpr int NST = I8 ? 1 : 2;
   constexpr int NJB = (kB * kB + 64 * MW - 1) / (64 * MW);
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
@@ -786,7 +803,12 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   // issues the same loads and the wait counts stay exact (a skipped load
   // would make the compiler wait for all of them).
   auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
-    const int jc = jn > 0 ? min(j, jn - 1) : nblk - 1 - c0;
+    // jn passes through an empty asm so that the per-stage offsets are
+    // recomputed (a few scalar ops) rather than hoisted out of the block
+    // loop into registers that would spill
+    int jnl = jn;
+    asm volatile("" : "+s"(jnl));
+    const int jc = jnl > 0 ? min(j, jnl - 1) : nblk - 1 - c0;
 #pragma unroll
     for (int st = 0; st < NST; ++st)
 #pragma unroll
@@ -841,11 +863,11 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
       for (int st = 0; st < NST; ++st) {
         uint32_t da[4], db[4];
         if (st == 0) {
-          spins_bf16<0>(nza, da);
-          spins_bf16<0>(nzb, db);
+          spins_bf16_r<0>(nza, vmask, vone, da);
+          spins_bf16_r<0>(nzb, vmask, vone, db);
         } else {
-          spins_bf16<1>(nza, da);
-          spins_bf16<1>(nzb, db);
+          spins_bf16_r<1>(nza, vmask, vone, da);
+          spins_bf16_r<1>(nzb, vmask, vone, db);
         }
         u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
         const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bbv = __builtin_bit_cast(bf16x8, ub);
@@ -885,6 +907,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   auto compute_part = [&](int cr, int ex, float* dst) {
     f32x16 acc0 = {}, acc1 = {};
     i32x16 iacc0 = {}, iacc1 = {};
+    const int jx = ex - c0;  // the skipped stage, if it is in this wave's range
     constexpr int kUnrollJ = NW <= 8 ? (2 * NW + PF - 1) / PF : 1;
 #pragma unroll kUnrollJ
     for (int jb0 = 0; jb0 < 2 * NW; jb0 += PF) {
@@ -892,22 +915,20 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int j = jb0 + u;
-        Frag fa[NST][NP];
-#pragma unroll
-        for (int st = 0; st < NST; ++st)
-#pragma unroll
-          for (int pc = 0; pc < NP; ++pc) fa[st][pc] = ring[u][st][pc];
-        load_stage(cr, j + PF, ring[u]);
+        // consume slot u, then refill it with stage j + PF (no copy of the
+        // fragments: the load is issued after the products that read them)
         if (j < jn) {
-          if (c0 + j == ex) {
+          if (j == jx) {
 #pragma unroll
             for (int st = 0; st < NST; ++st)
 #pragma unroll
-              for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = fa[st][pc];
+              for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = ring[u][st][pc];
           } else {
-            mfma_step(fa, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
+            mfma_step(ring[u], get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0,
+                      iacc1);
           }
         }
+        load_stage(cr, j + PF, ring[u]);
       }
     }
     write_fields(dst, acc0, acc1, iacc0, iacc1);
@@ -1219,10 +1240,6 @@ hipError_t launch_kw(const BParams& p, int nw, int groups, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Matrix waves of the pipelined kernel.  Four waves per workgroup keep one
-// wave per SIMD and its 512-VGPR budget; more waves would cap each at 256
-// registers, which the proposal wave's 32 fields and brackets exceed.
-constexpr int kPipeMW = 3;
 
 template <int NP, bool I8>
 hipError_t launch_pipe(const BParams& p, int nw, int groups, hipStream_t st) {
