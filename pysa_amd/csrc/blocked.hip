@@ -42,6 +42,7 @@
 #include <string.h>
 
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/pysa_amd.h"
@@ -203,6 +204,18 @@ __device__ __forceinline__ void accept_bounds(uint32_t w, float beta, float inv_
 // The exact test (pysa/ising.py:52) for the rare dn inside the bracket.
 __device__ __noinline__ bool accept_exact(uint32_t w, float beta, float dn) {
   return dn >= 0.f || log_u32(w) < (double)beta * (double)dn;
+}
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}):
+// a loop whose index is a compile-time constant in the body, so arrays indexed
+// by expressions of it stay in registers.
+template <typename F, int... K>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 template <int NW>
@@ -706,7 +719,8 @@ template <int MW>
 struct __align__(16) PipeLds {
   float part[2][MW][64 * 32];  // [parity][matrix wave][fld(replica, row)]
   float corr[64 * 32];         // J[c+1, c] . S_c of the next block
-  float jb[2][kB * kB];       // J^T[pos0+i][pos0+k] at i*32 + k
+  float jb[2][kB * kB];       // row i of J[blk][blk] right of the diagonal, shifted left:
+                              // J^T[pos0+i][pos0+i+1+t] at i*32 + t (t < 31 - i), else 0
   float hb[2][kB];
   float rsb[2][kB];
   float lo[2][kB][64];        // acceptance brackets [parity][row][replica]
@@ -823,8 +837,8 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
       const int e = mt + e2 * 64 * MW;
-      const int i = e >> 5, k = e & 31;
-      jb_reg[e2] = (e < kB * kB && pos0 + i < n && pos0 + k < n)
+      const int i = e >> 5, k = i + 1 + (e & 31);  // slot t = e & 31 holds column i + 1 + t
+      jb_reg[e2] = (e < kB * kB && k < kB && pos0 + i < n && pos0 + k < n)
                        ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
     }
     h_reg = (mt >= 0 && mt < kB && pos0 + mt < n) ? p.h[pos0 + mt] : 0.f;
@@ -976,17 +990,24 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   __syncthreads();
 
   const float cu = (p.ptype == 0) ? 2.f : 1.f;  // |dn / (F + h)|
+  // F = fsc * (sum of the partial products + rsb) + h: QUBO with bf16
+  // operands used +-1 spins, F = (J.s + J.1) / 2 with the row sums J.1 in
+  // rsb (zero otherwise); fsc is a power of two, so the fma rounds like the add
+  const bool qsum = !I8 && p.ptype == 1;
+  const float fsc = qsum ? 0.5f : 1.f;
   uint32_t t = 0;                               // running block counter
-  for (int s = 0; s < p.S; ++s) {
-    const uint64_t sw = (uint64_t)s + p.sweep_offset;
-    const float beta = act ? L.rb[q] : 0.f;
-    double dsum = 0.0;
-    uint32_t nacc = 0;
-    for (int c = 0; c < nblk; ++c, ++t) {
-      const int pb = t & 1, pn = pb ^ 1;
-      const int c1 = (c + 1 == nblk) ? 0 : c + 1;
-      const int pos0 = 32 * c;
-      if (w == 0) {
+  // The two roles run separate copies of the block loop (so that each keeps
+  // only its own state in registers); both pass the same barriers: one after
+  // the block's work (B2), two more at the end of a sweep, one per block (B3).
+  if (w == 0) {
+    for (int s = 0; s < p.S; ++s) {
+      const uint64_t sw = (uint64_t)s + p.sweep_offset;
+      const float beta = act ? L.rb[q] : 0.f;
+      double dsum = 0.0;
+      uint32_t nacc = 0;
+      for (int c = 0; c < nblk; ++c, ++t) {
+        const int pb = t & 1;
+        const int pos0 = 32 * c;
         // ---- proposals of block c, one lane per replica
         float fp[kB];
         {
@@ -1003,55 +1024,65 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
               sm.w = sm.w + a.w;
             }
             const float4 cr = *reinterpret_cast<const float4*>(&L.corr[o]);
-            const float vs[4] = {sm.x + cr.x, sm.y + cr.y, sm.z + cr.z, sm.w + cr.w};
+            const float4 h4 = *reinterpret_cast<const float4*>(&L.hb[pb][4 * r4]);
+            const float4 s4 = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * r4]);
+            const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y, sm.z + cr.z + s4.z,
+                                 sm.w + cr.w + s4.w};
+            const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int r = 4 * r4 + e;
-              float v = vs[e];
-              // QUBO with bf16 operands: the product used +-1 spins, F = (J.s + J.1) / 2
-              if (!I8 && p.ptype == 1) v = 0.5f * (v + L.rsb[pb][r]);
-              fp[r] = v + L.hb[pb][r];
-            }
+            for (int e = 0; e < 4; ++e) fp[4 * r4 + e] = __builtin_fmaf(fsc, vs[e], hv[e]);
           }
         }
         uint32_t mb = L.bb[pb][q];
         const uint32_t mb0 = mb;
+        // bit k of mb is only changed at step k, so every step reads its
+        // spin from the block's initial bits: sign bit = !spin, magnitude cu
+        const uint32_t nmb = ~mb0, cub = __float_as_uint(cu);
         float dpart = 0.f;
         bool amb = false;
         // Rows of J[blk][blk] (the same for every lane) stream from LDS
-        // through a ring JR rows ahead of the chain.
+        // through a ring JR rows ahead of the chain.  Row k holds only the
+        // columns right of the diagonal, shifted to the start of the row, so
+        // the ceil((31 - k) / 4) 16-byte loads of a row carry no dead lanes.
         constexpr int JR = 3;
         const float4* jb4 = reinterpret_cast<const float4*>(L.jb[pb]);
         float4 jring[JR][kB / 4];
-#pragma unroll
-        for (int d = 0; d < JR; ++d)
-#pragma unroll
-          for (int r4 = 0; r4 < kB / 4; ++r4) jring[d][r4] = jb4[d * (kB / 4) + r4];
-#pragma unroll
-        for (int k = 0; k < kB; ++k) {
-          const bool up = (mb >> k) & 1u;
+        static_for<JR>([&](auto dc) __attribute__((always_inline)) {
+          constexpr int d = decltype(dc)::value;
+          static_for<(kB - 1 - d + 3) / 4>([&](auto tc) __attribute__((always_inline)) {
+            constexpr int t4 = decltype(tc)::value;
+            jring[d][t4] = jb4[d * (kB / 4) + t4];
+          });
+        });
+        static_for<kB>([&](auto kc) __attribute__((always_inline)) {
+          constexpr int k = decltype(kc)::value;
           // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
-          const float sg = up ? cu : -cu;
+          const float sg = __uint_as_float(((nmb << (31 - k)) & 0x80000000u) | cub);
           const float dn = fp[k] * sg;
           const float hik = L.hi[pb][k][q], lok = L.lo[pb][k][q];
           const bool take = act & (dn > hik);
           amb |= act & !take & !(dn < lok);
           mb ^= (uint32_t)take << k;
-          dpart += take ? -dn : 0.f;
           const float dS = take ? -sg : 0.f;
+          // dS * fp[k] is exactly -dn or 0, so this rounds like dpart - dn
+          dpart = __builtin_fmaf(dS, fp[k], dpart);
+          static_for<(kB - 1 - k + 3) / 4>([&](auto tc) __attribute__((always_inline)) {
+            constexpr int t4 = decltype(tc)::value;
+            const float4 j4 = jring[k % JR][t4];
+            const float jv[4] = {j4.x, j4.y, j4.z, j4.w};
 #pragma unroll
-          for (int r4 = 0; r4 < kB / 4; ++r4) {
-            const float4 j4 = jring[k % JR][r4];
-            // dS is 0, +-1 or +-2, so the fused form rounds like mul + add
-            fp[4 * r4] = __builtin_fmaf(j4.x, dS, fp[4 * r4]);
-            fp[4 * r4 + 1] = __builtin_fmaf(j4.y, dS, fp[4 * r4 + 1]);
-            fp[4 * r4 + 2] = __builtin_fmaf(j4.z, dS, fp[4 * r4 + 2]);
-            fp[4 * r4 + 3] = __builtin_fmaf(j4.w, dS, fp[4 * r4 + 3]);
-          }
+            for (int e = 0; e < 4; ++e) {
+              const int r = k + 1 + 4 * t4 + e;
+              // dS is 0, +-1 or +-2, so the fused form rounds like mul + add
+              if (r < kB) fp[r] = __builtin_fmaf(jv[e], dS, fp[r]);
+            }
+          });
           if (k + JR < kB)
-#pragma unroll
-            for (int r4 = 0; r4 < kB / 4; ++r4) jring[k % JR][r4] = jb4[(k + JR) * (kB / 4) + r4];
-        }
+            static_for<(kB - 1 - (k + JR) + 3) / 4>([&](auto tc) __attribute__((always_inline)) {
+              constexpr int t4 = decltype(tc)::value;
+              jring[k % JR][t4] = jb4[(k + JR) * (kB / 4) + t4];
+            });
+        });
         if (__builtin_expect(__any(amb), 0)) {
           // A decision fell inside its bracket: rerun the block, taking the
           // exact test for the decisions inside their brackets (rolled loop,
@@ -1061,8 +1092,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
             float v = L.part[pb][0][fld(q, r)];
             for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(q, r)];
             v = v + L.corr[fld(q, r)];
-            if (!I8 && p.ptype == 1) v = 0.5f * (v + L.rsb[pb][r]);
-            sf[r] = v + L.hb[pb][r];
+            sf[r] = __builtin_fmaf(fsc, v + L.rsb[pb][r], L.hb[pb][r]);
           }
           mb = mb0;
           dpart = 0.f;
@@ -1080,35 +1110,16 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
             mb ^= (uint32_t)take << k;
             dpart += take ? -dn : 0.f;
             const float dS = take ? -sg : 0.f;
-            for (int r = 0; r < kB; ++r) sf[r] = __builtin_fmaf(L.jb[pb][k * kB + r], dS, sf[r]);
+            for (int r = k + 1; r < kB; ++r)
+              sf[r] = __builtin_fmaf(L.jb[pb][k * kB + (r - k - 1)], dS, sf[r]);
           }
         }
         L.bnew[q] = mb;
         dsum += (double)dpart;
         nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
-      } else {
-        // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
-        publish_block(pn);
-        const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
-        compute_part(c1, c, L.part[pn][m]);
-        prefetch_block(c2);
-        if (c1 != 0) brackets(c1, sw, pn);
-      }
-      __syncthreads();
-      if (mw && w == owner_of(c)) {
-        // ---- the owner of block c takes its new spins and adds J[c1, c] . S_c
-        const int j = c - c0;
-        set_half<NW>(sa, j, half_of<I8>(L.bnew[qa], h));
-        set_half<NW>(sb, j, half_of<I8>(L.bnew[qb], h));
-        f32x16 acc0 = {}, acc1 = {};
-        i32x16 iacc0 = {}, iacc1 = {};
-        mfma_step(fcorr, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
-        write_fields(L.corr, acc0, acc1, iacc0, iacc1);
-      }
-      if (mw && w == owner_of(c1)) publish_bits(c1, pn);
-      if (c + 1 == nblk) {
-        // ---- end of sweep: energy (simulation.py:157-158), best states, PT
-        if (w == 0) {
+        __syncthreads();  // B2
+        if (c + 1 == nblk) {
+          // ---- end of sweep: energy (simulation.py:157-158), best states, PT
           acc += nacc;
           E = (float)((double)E + dsum);
           L.E[q] = E;
@@ -1124,65 +1135,97 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
           if (p.use_pt && act && rr < R - 1)
             L.lu[q] = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)rr, p.k0, p.k1),
                                         (uint32_t)rr & 3u));
-        }
-        __syncthreads();
-        if (mw && p.best_sem == 0) {
-          if (acta && L.flag[qa]) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_a);
-          if (actb && L.flag[qb]) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_rep + base_b);
-        }
-        if (w == 0 && act && rr == 0) {  // read leader
-          const int b0 = g * R;
-          int src = -1;
-          if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
-            float cur = gE;
-            for (int k = 0; k < R; ++k)
-              if (L.E[b0 + k] < cur) {
-                cur = L.E[b0 + k];
-                src = k;
-              }
-            if (src >= 0) {
-              gE = cur;
-              gsw = s;
-              gimp = 1;
-            }
-          }
-          if (p.use_pt) {  // utils.py:44-58
-            for (int k = 0; k < R - 1; ++k) {
-              const int a = R - k - 1, b = R - k - 2;
-              const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
-                               (p.betas[a] - p.betas[b]);
-              if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
-                const int tt = L.bi[b0 + a];
-                L.bi[b0 + a] = L.bi[b0 + b];
-                L.bi[b0 + b] = tt;
+          __syncthreads();
+          if (act && rr == 0) {  // read leader
+            const int b0 = g * R;
+            int src = -1;
+            if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
+              float cur = gE;
+              for (int k = 0; k < R; ++k)
+                if (L.E[b0 + k] < cur) {
+                  cur = L.E[b0 + k];
+                  src = k;
+                }
+              if (src >= 0) {
+                gE = cur;
+                gsw = s;
+                gimp = 1;
               }
             }
-          }
-          for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = p.betas[k];
-          if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
-            double lw = 0.0, bold = 0.0;
-            for (int r = 1; r < R; ++r) {
-              const int k = p.bsort[r];
-              const float be = p.betas[k];
-              const float en = L.E[b0 + L.bi[b0 + k]];
-              const float a = -be * en;
-              lw += (double)a - (-bold * (double)en);
-              bold = (double)be;
+            if (p.use_pt) {  // utils.py:44-58
+              for (int k = 0; k < R - 1; ++k) {
+                const int a = R - k - 1, b = R - k - 2;
+                const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
+                                 (p.betas[a] - p.betas[b]);
+                if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
+                  const int tt = L.bi[b0 + a];
+                  L.bi[b0 + a] = L.bi[b0 + b];
+                  L.bi[b0 + b] = tt;
+                }
+              }
             }
-            p.log_omegas[(size_t)read * p.S + s] = lw;
+            for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = p.betas[k];
+            if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
+              double lw = 0.0, bold = 0.0;
+              for (int r = 1; r < R; ++r) {
+                const int k = p.bsort[r];
+                const float be = p.betas[k];
+                const float en = L.E[b0 + L.bi[b0 + k]];
+                const float a = -be * en;
+                lw += (double)a - (-bold * (double)en);
+                bold = (double)be;
+              }
+              p.log_omegas[(size_t)read * p.S + s] = lw;
+            }
+            L.src[g] = src;
           }
-          L.src[g] = src;
+          __syncthreads();
         }
-        __syncthreads();
-        if (mw && p.best_sem == 1) {  // owners copy their part of a new global best
-          if (acta && L.src[ga] == qa - ga * R)
-            store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
-          if (actb && L.src[gb] == qb - gb * R)
-            store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
-        }
-        if (mw) brackets(0, sw + 1, pn);  // next sweep's first block, with the new betas
+        __syncthreads();  // B3
       }
-      __syncthreads();
+    }
+  } else {
+    for (int s = 0; s < p.S; ++s) {
+      const uint64_t sw = (uint64_t)s + p.sweep_offset;
+      for (int c = 0; c < nblk; ++c, ++t) {
+        const int pb = t & 1, pn = pb ^ 1;
+        const int c1 = (c + 1 == nblk) ? 0 : c + 1;
+        (void)pb;
+        // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
+        publish_block(pn);
+        const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
+        compute_part(c1, c, L.part[pn][m]);
+        prefetch_block(c2);
+        if (c1 != 0) brackets(c1, sw, pn);
+        __syncthreads();  // B2
+        if (w == owner_of(c)) {
+          // ---- the owner of block c takes its new spins and adds J[c1, c] . S_c
+          const int j = c - c0;
+          set_half<NW>(sa, j, half_of<I8>(L.bnew[qa], h));
+          set_half<NW>(sb, j, half_of<I8>(L.bnew[qb], h));
+          f32x16 acc0 = {}, acc1 = {};
+          i32x16 iacc0 = {}, iacc1 = {};
+          mfma_step(fcorr, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
+          write_fields(L.corr, acc0, acc1, iacc0, iacc1);
+        }
+        if (w == owner_of(c1)) publish_bits(c1, pn);
+        if (c + 1 == nblk) {
+          __syncthreads();
+          if (p.best_sem == 0) {
+            if (acta && L.flag[qa]) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_a);
+            if (actb && L.flag[qb]) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_rep + base_b);
+          }
+          __syncthreads();
+          if (p.best_sem == 1) {  // owners copy their part of a new global best
+            if (acta && L.src[ga] == qa - ga * R)
+              store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
+            if (actb && L.src[gb] == qb - gb * R)
+              store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
+          }
+          brackets(0, sw + 1, pn);  // next sweep's first block, with the new betas
+        }
+        __syncthreads();  // B3
+      }
     }
   }
 
