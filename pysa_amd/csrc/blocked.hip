@@ -737,48 +737,47 @@ struct __align__(16) PipeLds {
 };
 
 template <int NP, bool I8, int NW, int MW>
-__global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) {
+__global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p) {
   __shared__ PipeLds<MW> L;
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tid = threadIdx.x;
   const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
   const int read0 = blockIdx.x * p.rpw;
   const int h = lane >> 5;
-  // Operand role (matrix waves): tile-0 replica qa and tile-1 replica qb.
-  const int qa = lane & 31, qb = 32 + (lane & 31);
-  const int ga = qa / R, gb = qb / R;
-  const bool acta = (ga < p.rpw) && (read0 + ga < p.reads);
-  const bool actb = (gb < p.rpw) && (read0 + gb < p.reads);
-  const size_t base_a = acta ? ((size_t)(read0 + ga) * R + (qa - ga * R)) * n : 0;
-  const size_t base_b = actb ? ((size_t)(read0 + gb) * R + (qb - gb * R)) * n : 0;
+  // Two waves per SIMD: slot wl = w & 3 and tile = w >> 2.  Wave 0 runs the
+  // proposals; matrix wave (m = wl - 1, tile) computes the products of K
+  // range m for the 32 replicas of its tile; wave 4 only keeps the barriers
+  // (it shares wave 0's SIMD).
+  const int wl = w & 3, tile = w >> 2;
+  const bool mw = wl != 0;  // a matrix wave
+  const int m = wl - 1;
+  const int mi = mw ? tile * MW + m : 2 * MW;  // matrix-wave index 0..2MW-1
+  // Operand role (matrix waves): replica qt of this wave's tile.
+  const int qt = tile * 32 + (lane & 31);
+  const int gt = qt / R;
+  const bool actt = mw && (gt < p.rpw) && (read0 + gt < p.reads);
+  const size_t base_t = actt ? ((size_t)(read0 + gt) * R + (qt - gt * R)) * n : 0;
   // Replica role (every wave): lane = replica q of the group.
   const int q = lane;
   const int g = q / R, rr = q - g * R;
   const int read = read0 + g;
   const bool act = (g < p.rpw) && (read < p.reads);
   const uint64_t cid = (uint64_t)read * R + rr;
-  const bool mw = w > 0;  // a matrix wave
-  const int m = w - 1;
   const int c0 = mw ? m * kb : 0;
   const int jn = mw ? max(0, min(kb, nblk - c0)) : 0;
-  const int mt = tid - 64;  // matrix-lane index 0..64*MW-1
+  const int mt = mw ? mi * 64 + lane : -1;  // matrix-lane index 0..128*MW-1
 
-  Bits<NW> sa, sb;
+  Bits<NW> sa;  // this wave's K range of its tile's spins, operand layout
   sa.w = 0;
-  sb.w = 0;
   for (int j = 0; j < jn; ++j) {
     const int c = c0 + j;
-    uint32_t va = 0, vb = 0;
+    uint32_t va = 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
       const int rel = I8 ? (16 * h + b) : ((b < 8) ? (8 * h + b) : (16 + 8 * h + (b - 8)));
       const int pos = 32 * c + rel;
-      if (pos < n) {
-        if (acta && p.spins[base_a + pos] == 1) va |= 1u << b;
-        if (actb && p.spins[base_b + pos] == 1) vb |= 1u << b;
-      }
+      if (pos < n && actt && p.spins[base_t + pos] == 1) va |= 1u << b;
     }
     set_half<NW>(sa, j, va);
-    set_half<NW>(sb, j, vb);
   }
   float E = 0.f, bestE = 0.f, gE = 0.f;
   int gsw = 0, gimp = 0;
@@ -794,7 +793,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
       p.best_rep_sw[(size_t)read * R + rr] = 0;
     }
   }
-  for (int i = tid; i < 64 * 32; i += 64 * (MW + 1)) L.corr[i] = 0.f;
+  for (int i = tid; i < 64 * 32; i += 128 * (MW + 1)) L.corr[i] = 0.f;
 
   constexpr int PF = kPipePF;
   const uint32_t emul = p.ptype == 0 ? 0xFEu : 1u;
@@ -803,7 +802,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   uint32_t vmask = 0x80008000u, vone = 0x3F803F80u;
   asm volatile("" : "+v"(vmask), "+v"(vone));
   constexpr int NST = I8 ? 1 : 2;
-  constexpr int NJB = (kB * kB + 64 * MW - 1) / (64 * MW);
+  constexpr int NJB = (kB * kB + 128 * MW - 1) / (128 * MW);
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
   Frag ring[PF][NST][NP], fcorr[NST][NP];
   float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
@@ -836,7 +835,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     const int pos0 = 32 * c;
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = mt + e2 * 64 * MW;
+      const int e = mt + e2 * 128 * MW;
       const int i = e >> 5, k = i + 1 + (e & 31);  // slot t = e & 31 holds column i + 1 + t
       jb_reg[e2] = (e < kB * kB && k < kB && pos0 + i < n && pos0 + k < n)
                        ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
@@ -847,80 +846,63 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   auto publish_block = [&](int pb) {
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = mt + e2 * 64 * MW;
-      if (e < kB * kB) L.jb[pb][e] = jb_reg[e2];
+      const int e = mt + e2 * 128 * MW;
+      if (mt >= 0 && e < kB * kB) L.jb[pb][e] = jb_reg[e2];
     }
     if (mt >= 0 && mt < kB) {
       L.hb[pb][mt] = h_reg;
       L.rsb[pb][mt] = rs_reg;
     }
   };
-  // Products of K step j (fragments fa) with spins (ha: tile 0, hb2: tile 1).
-  auto mfma_step = [&](const Frag (&fa)[NST][NP], uint32_t ha, uint32_t hb2, f32x16& acc0,
-                       f32x16& acc1, i32x16& iacc0, i32x16& iacc1) {
+  // Products of K step j (fragments fa) with this tile's spins hx.
+  auto mfma_step = [&](const Frag (&fa)[NST][NP], uint32_t hx, f32x16& acc, i32x16& iacc) {
     if (I8) {
-      uint32_t da[4], db[4];
-      spins_i8(ha, emul, exr, da);
-      spins_i8(hb2, emul, exr, db);
-      i32x4 ba, bbv;
+      uint32_t d[4];
+      spins_i8(hx, emul, exr, d);
+      i32x4 bv;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        ba[e] = (int)da[e];
-        bbv[e] = (int)db[e];
-      }
+      for (int e = 0; e < 4; ++e) bv[e] = (int)d[e];
       const i32x4 a8 = __builtin_bit_cast(i32x4, fa[0][0]);
-      iacc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, ba, iacc0, 0, 0, 0);
-      iacc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, bbv, iacc1, 0, 0, 0);
+      iacc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, bv, iacc, 0, 0, 0);
     } else {
-      const uint32_t nza = spins_nz(ha), nzb = spins_nz(hb2);
+      const uint32_t nz = spins_nz(hx);
 #pragma unroll
       for (int st = 0; st < NST; ++st) {
-        uint32_t da[4], db[4];
-        if (st == 0) {
-          spins_bf16_r<0>(nza, vmask, vone, da);
-          spins_bf16_r<0>(nzb, vmask, vone, db);
-        } else {
-          spins_bf16_r<1>(nza, vmask, vone, da);
-          spins_bf16_r<1>(nzb, vmask, vone, db);
-        }
-        u32x4v ua = {da[0], da[1], da[2], da[3]}, ub = {db[0], db[1], db[2], db[3]};
-        const bf16x8 ba = __builtin_bit_cast(bf16x8, ua), bbv = __builtin_bit_cast(bf16x8, ub);
+        uint32_t d[4];
+        if (st == 0)
+          spins_bf16_r<0>(nz, vmask, vone, d);
+        else
+          spins_bf16_r<1>(nz, vmask, vone, d);
+        u32x4v ud = {d[0], d[1], d[2], d[3]};
+        const bf16x8 bv = __builtin_bit_cast(bf16x8, ud);
 #pragma unroll
-        for (int pc = 0; pc < NP; ++pc) {
-          const bf16x8 a16 = __builtin_bit_cast(bf16x8, fa[st][pc]);
-          acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16, ba, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a16, bbv, acc1, 0, 0, 0);
-        }
+        for (int pc = 0; pc < NP; ++pc)
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, fa[st][pc]), bv,
+                                                        acc, 0, 0, 0);
       }
     }
   };
   // Fields from the C layout (register v: row 8(v/4) + 4h + v%4 of column
   // lane%32) into [replica][row], four consecutive rows per 16-byte store.
-  auto write_fields = [&](float* dst, const f32x16& acc0, const f32x16& acc1, const i32x16& iacc0,
-                          const i32x16& iacc1) {
+  auto write_fields = [&](float* dst, const f32x16& acc, const i32x16& iacc) {
 #pragma unroll
     for (int g4 = 0; g4 < 4; ++g4) {
       const int i = 8 * g4 + 4 * h;
-      float4 x, y;
-      if (I8) {
-        x = make_float4((float)iacc0[4 * g4], (float)iacc0[4 * g4 + 1], (float)iacc0[4 * g4 + 2],
-                        (float)iacc0[4 * g4 + 3]);
-        y = make_float4((float)iacc1[4 * g4], (float)iacc1[4 * g4 + 1], (float)iacc1[4 * g4 + 2],
-                        (float)iacc1[4 * g4 + 3]);
-      } else {
-        x = make_float4(acc0[4 * g4], acc0[4 * g4 + 1], acc0[4 * g4 + 2], acc0[4 * g4 + 3]);
-        y = make_float4(acc1[4 * g4], acc1[4 * g4 + 1], acc1[4 * g4 + 2], acc1[4 * g4 + 3]);
-      }
-      *reinterpret_cast<float4*>(&dst[fld(lane & 31, i)]) = x;
-      *reinterpret_cast<float4*>(&dst[fld(32 + (lane & 31), i)]) = y;
+      float4 x;
+      if (I8)
+        x = make_float4((float)iacc[4 * g4], (float)iacc[4 * g4 + 1], (float)iacc[4 * g4 + 2],
+                        (float)iacc[4 * g4 + 3]);
+      else
+        x = make_float4(acc[4 * g4], acc[4 * g4 + 1], acc[4 * g4 + 2], acc[4 * g4 + 3]);
+      *reinterpret_cast<float4*>(&dst[fld(qt, i)]) = x;
     }
   };
   // Partial fields of row block cr over this wave's K range, skipping K block
   // ex (its fragments are kept in fcorr for the correction); ring holds the
   // first PF stages of row block cr on entry.
   auto compute_part = [&](int cr, int ex, float* dst) {
-    f32x16 acc0 = {}, acc1 = {};
-    i32x16 iacc0 = {}, iacc1 = {};
+    f32x16 acc = {};
+    i32x16 iacc = {};
     const int jx = ex - c0;  // the skipped stage, if it is in this wave's range
     constexpr int kUnrollJ = NW <= 8 ? (2 * NW + PF - 1) / PF : 1;
 #pragma unroll kUnrollJ
@@ -938,14 +920,13 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
 #pragma unroll
               for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = ring[u][st][pc];
           } else {
-            mfma_step(ring[u], get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0,
-                      iacc1);
+            mfma_step(ring[u], get_half<NW>(sa, j), acc, iacc);
           }
         }
         load_stage(cr, j + PF, ring[u]);
       }
     }
-    write_fields(dst, acc0, acc1, iacc0, iacc1);
+    write_fields(dst, acc, iacc);
   };
   // Acceptance brackets of row block cr for sweep counter sw: this matrix
   // wave's share of the rows, lane = replica (beta from L.rb).
@@ -954,7 +935,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     const float inv_b = (beta > 0.f) ? 1.0f / beta : 0.f;
     const int pos0 = 32 * cr;
     // groups of 4 rows (one Philox block each) g = m, m + MW, ...
-    for (int g4 = m; g4 < kB / 4; g4 += MW) {
+    for (int g4 = mi; g4 < kB / 4; g4 += 2 * MW) {
       const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
       const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
@@ -969,12 +950,11 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     }
   };
   auto owner_of = [&](int c) { return 1 + c / kb; };  // wave owning K block c
-  auto publish_bits = [&](int c, int pb) {  // by the owner of block c
+  auto publish_bits = [&](int c, int pb) {  // by the owners of block c (both tiles)
     const int j = c - c0;
-    const uint32_t mine = (lane < 32) ? get_half<NW>(sa, j) : get_half<NW>(sb, j);
-    const uint32_t give = (lane < 32) ? get_half<NW>(sb, j) : get_half<NW>(sa, j);
-    const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, 64);
-    L.bb[pb][lane] = (lane < 32) ? block_of<I8>(mine, got) : block_of<I8>(got, mine);
+    const uint32_t mine = get_half<NW>(sa, j);
+    const uint32_t got = (uint32_t)__shfl_xor((int)mine, 32, 64);  // the other half
+    if (lane < 32) L.bb[pb][qt] = block_of<I8>(mine, got);
   };
 
   // ---- prologue: the fields of block 0 in full, its brackets and spins
@@ -985,7 +965,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
     compute_part(0, -1, L.part[0][m]);
     prefetch_block(nblk > 1 ? 1 : 0);
     brackets(0, sw0, 0);
-    if (w == owner_of(0)) publish_bits(0, 0);
+    if (wl == owner_of(0)) publish_bits(0, 0);
   }
   __syncthreads();
 
@@ -1044,7 +1024,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
         // through a ring JR rows ahead of the chain.  Row k holds only the
         // columns right of the diagonal, shifted to the start of the row, so
         // the ceil((31 - k) / 4) 16-byte loads of a row carry no dead lanes.
-        constexpr int JR = 3;
+        constexpr int JR = 2;
         const float4* jb4 = reinterpret_cast<const float4*>(L.jb[pb]);
         float4 jring[JR][kB / 4];
         static_for<JR>([&](auto dc) __attribute__((always_inline)) {
@@ -1192,37 +1172,33 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
         const int c1 = (c + 1 == nblk) ? 0 : c + 1;
         (void)pb;
         // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
-        publish_block(pn);
-        const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
-        compute_part(c1, c, L.part[pn][m]);
-        prefetch_block(c2);
-        if (c1 != 0) brackets(c1, sw, pn);
-        __syncthreads();  // B2
-        if (w == owner_of(c)) {
-          // ---- the owner of block c takes its new spins and adds J[c1, c] . S_c
-          const int j = c - c0;
-          set_half<NW>(sa, j, half_of<I8>(L.bnew[qa], h));
-          set_half<NW>(sb, j, half_of<I8>(L.bnew[qb], h));
-          f32x16 acc0 = {}, acc1 = {};
-          i32x16 iacc0 = {}, iacc1 = {};
-          mfma_step(fcorr, get_half<NW>(sa, j), get_half<NW>(sb, j), acc0, acc1, iacc0, iacc1);
-          write_fields(L.corr, acc0, acc1, iacc0, iacc1);
+        if (mw) {
+          publish_block(pn);
+          const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
+          compute_part(c1, c, L.part[pn][m]);
+          prefetch_block(c2);
+          if (c1 != 0) brackets(c1, sw, pn);
         }
-        if (w == owner_of(c1)) publish_bits(c1, pn);
+        __syncthreads();  // B2
+        if (mw && wl == owner_of(c)) {
+          // ---- the owners of block c take its new spins and add J[c1, c] . S_c
+          const int j = c - c0;
+          set_half<NW>(sa, j, half_of<I8>(L.bnew[qt], h));
+          f32x16 acc = {};
+          i32x16 iacc = {};
+          mfma_step(fcorr, get_half<NW>(sa, j), acc, iacc);
+          write_fields(L.corr, acc, iacc);
+        }
+        if (mw && wl == owner_of(c1)) publish_bits(c1, pn);
         if (c + 1 == nblk) {
           __syncthreads();
-          if (p.best_sem == 0) {
-            if (acta && L.flag[qa]) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_a);
-            if (actb && L.flag[qb]) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_rep + base_b);
-          }
+          if (mw && p.best_sem == 0 && actt && L.flag[qt])
+            store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_t);
           __syncthreads();
-          if (p.best_sem == 1) {  // owners copy their part of a new global best
-            if (acta && L.src[ga] == qa - ga * R)
-              store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
-            if (actb && L.src[gb] == qb - gb * R)
-              store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
-          }
-          brackets(0, sw + 1, pn);  // next sweep's first block, with the new betas
+          // owners copy their part of a new global best
+          if (mw && p.best_sem == 1 && actt && L.src[gt] == qt - gt * R)
+            store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gt) * n);
+          if (mw) brackets(0, sw + 1, pn);  // next sweep's first block, with the new betas
         }
         __syncthreads();  // B3
       }
@@ -1230,10 +1206,7 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   }
 
   // ---- outputs
-  if (mw) {
-    if (acta) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.spins + base_a);
-    if (actb) store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.spins + base_b);
-  }
+  if (actt) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.spins + base_t);
   if (w == 0 && act) {
     if (p.accepted) atomicAdd(p.accepted + read, acc);
     p.energies[(size_t)read * R + rr] = E;
@@ -1247,12 +1220,8 @@ __global__ __launch_bounds__(64 * (MW + 1)) void blocked_pipe_kernel(BParams p) 
   if (p.best_sem == 1) {
     // never improved: best_state is still the view of states[0] (simulation.py:204)
     __syncthreads();
-    if (mw) {
-      if (acta && qa - ga * R == 0 && !L.src[ga])
-        store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + ga) * n);
-      if (actb && qb - gb * R == 0 && !L.src[gb])
-        store_range<NW, I8>(sb, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gb) * n);
-    }
+    if (actt && qt - gt * R == 0 && !L.src[gt])
+      store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gt) * n);
   }
 }
 
@@ -1286,7 +1255,7 @@ hipError_t launch_kw(const BParams& p, int nw, int groups, hipStream_t st) {
 
 template <int NP, bool I8>
 hipError_t launch_pipe(const BParams& p, int nw, int groups, hipStream_t st) {
-  dim3 grid(groups), block(64 * (kPipeMW + 1));
+  dim3 grid(groups), block(128 * (kPipeMW + 1));
   switch (nw) {
     case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW>), grid, block, 0, st, p); break;
     case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW>), grid, block, 0, st, p); break;
