@@ -934,8 +934,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     const float beta = L.rb[q];
     const float inv_b = (beta > 0.f) ? 1.0f / beta : 0.f;
     const int pos0 = 32 * cr;
-    // groups of 4 rows (one Philox block each) g = m, m + MW, ...
-    for (int g4 = mi; g4 < kB / 4; g4 += 2 * MW) {
+    // groups of 4 rows (one Philox block each): one per matrix wave, the
+    // rest to wave 4 (which shares a SIMD with the proposal wave only)
+    const int gstart = mw ? mi : 2 * MW, gstep = mw ? kB / 4 : 1;
+    for (int g4 = gstart; g4 < kB / 4; g4 += gstep) {
       const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
       const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
@@ -958,15 +960,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   };
 
   // ---- prologue: the fields of block 0 in full, its brackets and spins
+  __syncthreads();  // wave 0's betas (L.rb) and the zeroed L.corr are visible
   const uint64_t sw0 = p.sweep_offset;
   if (mw) {
     prefetch_block(0);
     publish_block(0);
     compute_part(0, -1, L.part[0][m]);
     prefetch_block(nblk > 1 ? 1 : 0);
-    brackets(0, sw0, 0);
     if (wl == owner_of(0)) publish_bits(0, 0);
   }
+  if (mw || w == 4) brackets(0, sw0, 0);
   __syncthreads();
 
   const float cu = (p.ptype == 0) ? 2.f : 1.f;  // |dn / (F + h)|
@@ -1177,8 +1180,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
           compute_part(c1, c, L.part[pn][m]);
           prefetch_block(c2);
-          if (c1 != 0) brackets(c1, sw, pn);
         }
+        if ((mw || w == 4) && c1 != 0) brackets(c1, sw, pn);
         __syncthreads();  // B2
         if (mw && wl == owner_of(c)) {
           // ---- the owners of block c take its new spins and add J[c1, c] . S_c
@@ -1198,7 +1201,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // owners copy their part of a new global best
           if (mw && p.best_sem == 1 && actt && L.src[gt] == qt - gt * R)
             store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gt) * n);
-          if (mw) brackets(0, sw + 1, pn);  // next sweep's first block, with the new betas
+          if (mw || w == 4) brackets(0, sw + 1, pn);  // next sweep's first block, new betas
         }
         __syncthreads();  // B3
       }
