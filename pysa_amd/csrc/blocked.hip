@@ -806,6 +806,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
   Frag ring[PF][NST][NP], fcorr[NST][NP];
   float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
+  uint32_t jb_ok = 0;  // bit e2: jb_reg[e2] is a coupling; bit 31: h_reg / rs_reg are
   auto frag_ptr = [&](int c, int j, int st, int pc) -> const Frag* {
     if (I8) return reinterpret_cast<const Frag*>(p.frag) + ((size_t)c * nblk + c0 + j) * 64 + lane;
     return reinterpret_cast<const Frag*>(p.frag) +
@@ -833,25 +834,33 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
 #pragma unroll
     for (int u = 0; u < PF; ++u) load_stage(c, u, ring[u]);
     const int pos0 = 32 * c;
+    // Every lane loads (an in-bounds address when its element is padding)
+    // and publish_block zeroes the padding: no register is written here
+    // except by these loads, which keeps them from waiting on older loads.
+    jb_ok = 0;
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
       const int e = mt + e2 * 128 * MW;
       const int i = e >> 5, k = i + 1 + (e & 31);  // slot t = e & 31 holds column i + 1 + t
-      jb_reg[e2] = (e < kB * kB && k < kB && pos0 + i < n && pos0 + k < n)
-                       ? p.Jt[(size_t)(pos0 + i) * n + pos0 + k] : 0.f;
+      const bool ok = e < kB * kB && k < kB && pos0 + i < n && pos0 + k < n;
+      jb_ok |= (uint32_t)ok << e2;
+      jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + i) * n + pos0 + k : 0];
     }
-    h_reg = (mt >= 0 && mt < kB && pos0 + mt < n) ? p.h[pos0 + mt] : 0.f;
-    rs_reg = (!I8 && p.rowsum && mt >= 0 && mt < kB && pos0 + mt < n) ? p.rowsum[pos0 + mt] : 0.f;
+    const bool hok = mt >= 0 && mt < kB && pos0 + mt < n;
+    jb_ok |= (uint32_t)hok << 31;
+    h_reg = p.h[hok ? pos0 + mt : 0];
+    if (!I8 && p.rowsum) rs_reg = p.rowsum[hok ? pos0 + mt : 0];
   };
   auto publish_block = [&](int pb) {
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
       const int e = mt + e2 * 128 * MW;
-      if (mt >= 0 && e < kB * kB) L.jb[pb][e] = jb_reg[e2];
+      if (mt >= 0 && e < kB * kB) L.jb[pb][e] = ((jb_ok >> e2) & 1u) ? jb_reg[e2] : 0.f;
     }
     if (mt >= 0 && mt < kB) {
-      L.hb[pb][mt] = h_reg;
-      L.rsb[pb][mt] = rs_reg;
+      const bool hok = (jb_ok >> 31) & 1u;
+      L.hb[pb][mt] = hok ? h_reg : 0.f;
+      L.rsb[pb][mt] = (hok && !I8 && p.rowsum) ? rs_reg : 0.f;
     }
   };
   // Products of K step j (fragments fa) with this tile's spins hx.
