@@ -816,6 +816,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   // range's last K block (recently read, so a cache hit), so every path
   // issues the same loads and the wait counts stay exact (a skipped load
   // would make the compiler wait for all of them).
+  // One K block of fragments: NST * NP * 64 of them, consecutive.
+  constexpr int kStage = NST * NP * 64;
   auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
     // jn passes through an empty asm so that the per-stage offsets are
     // recomputed (a few scalar ops) rather than hoisted out of the block
@@ -823,10 +825,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     int jnl = jn;
     asm volatile("" : "+s"(jnl));
     const int jc = jnl > 0 ? min(j, jnl - 1) : nblk - 1 - c0;
+    // wave-uniform base of the stage, then a per-lane 16-byte offset
+    const Frag* base = frag_ptr(c, 0, 0, 0) - lane + (size_t)jc * kStage;
 #pragma unroll
     for (int st = 0; st < NST; ++st)
 #pragma unroll
-      for (int pc = 0; pc < NP; ++pc) dst[st][pc] = *frag_ptr(c, jc, st, pc);
+      for (int pc = 0; pc < NP; ++pc) dst[st][pc] = base[(st * NP + pc) * 64 + lane];
   };
   // Request row block c: the first PF fragment stages of this wave's K
   // range and (spread over the matrix lanes) J[blk][blk], h and row sums.
