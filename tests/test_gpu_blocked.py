@@ -20,13 +20,13 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("ptype", [0, 1])
 @pytest.mark.parametrize("n,R,reads", [(32, 16, 4), (64, 16, 5), (100, 8, 9), (257, 32, 3),
-                                       (1024, 32, 2)])
+                                       (1024, 32, 2), (2048, 24, 3)])
 @pytest.mark.parametrize("kind", ["pm1", "int"])
 def test_blocked_bit_exact_vs_oracle(ptype, n, R, reads, kind):
     rng = np.random.default_rng(2000 + n + 7 * R + ptype)
     J, h = pm1_problem(n, rng, h_range=1) if kind == "pm1" else int_problem(n, rng, -9, 9, 3)
     J, h = J.astype(np.float32), h.astype(np.float32)
-    S = 30 if n < 1000 else 6
+    S = 30 if n < 1000 else (6 if n < 2000 else 3)
     betas = (1 / np.geomspace(0.8, 4.0, R)).astype(np.float32)
     st = random_spins((reads, R, n), rng, ptype)
     en = energies_of(J, h, st, ptype, np.float32)
