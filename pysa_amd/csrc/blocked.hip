@@ -709,7 +709,7 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
 // wave per SIMD and its 512-VGPR budget; more waves would cap each at 256
 // registers, which the proposal wave's 32 fields and brackets exceed.
 constexpr int kPipeMW = 3;
-constexpr int kPipePF = 4;  // fragment stages in flight per matrix wave (slot ring)
+constexpr int kPipePF = 3;  // fragment stages in flight per matrix wave (slot ring)
 
 // A replica's 32 fields take 32 floats; its 16-byte row groups are XOR-
 // swizzled by the replica index so that 8 lanes of a 16-byte access hit all
@@ -925,8 +925,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       for (int u = 0; u < PF; ++u) {
         const int j = jb0 + u;
         // consume slot u, then refill it with stage j + PF (no copy of the
-        // fragments: the load is issued after the products that read them)
-        if (j < jn) {
+        // fragments: the load is issued after the products that read them);
+        // slots past the 2 NW spin halves exist only when PF does not divide
+        // 2 NW and are never live
+        if (j < 2 * NW && j < jn) {
           if (j == jx) {
 #pragma unroll
             for (int st = 0; st < NST; ++st)
