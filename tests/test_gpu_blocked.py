@@ -67,3 +67,22 @@ def test_blocked_float_couplings(form):
     assert rel.max() < 1e-5, rel.max()
     b64 = oracle.energy_f64(J, h, got["best_state"])
     assert np.allclose(got["best_energy"], b64, rtol=1e-5)
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+def test_blocked_bf16_pieces_integer_bit_exact(ptype):
+    """Integer couplings outside int8 take the bf16 path (hi + lo pieces, and
+    for QUBO the +-1 operands with device row sums); their sums are exact, so
+    the trajectories still equal the oracle's bit for bit."""
+    rng = np.random.default_rng(31 + ptype)
+    n, R, reads, S = 200, 16, 3, 20
+    J, h = int_problem(n, rng, -300, 300, 40)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    betas = (1 / np.geomspace(20.0, 400.0, R)).astype(np.float32)
+    st = random_spins((reads, R, n), rng, ptype)
+    en = energies_of(J, h, st, ptype, np.float32)
+    ref = oracle.metropolis(J, h, ptype, betas, st, en, S, order=0, seed=91, nthreads=8)
+    got = engine_metropolis(J, h, ptype, betas, st, en, S, order=0, seed=91,
+                            form=_lib.FORM_BLOCKED)
+    assert ref["accepted"].sum() > 0
+    assert_same(got, ref)
