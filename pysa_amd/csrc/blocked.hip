@@ -201,6 +201,39 @@ __device__ __forceinline__ void accept_bounds(uint32_t w, float beta, float inv_
   lo = q - m;
 }
 
+// The same bracket with the per-replica factors taken out of the per-row
+// work: q = log2(uf) * cb with cb = ln2 / beta (one rounding fewer than
+// L = log2(uf) ln2 followed by q = L / beta, the same count overall), and
+// m = |q| A + c0 with A = 4.27e-6 >= 1.001 * 3.9e-6 (1 + 2^-22) + 3.6e-7,
+// c0 = 9.62e-7 / beta >= 1.001 * 9.6e-7 / beta, both rounded up so that the
+// fma's own rounding stays inside the margin of accept_bounds.  beta <= 0
+// gives cb = 0, c0 = -inf: hi = -inf, every proposal taken; beta too small
+// to scale gives cb = 0, c0 = +inf: every decision exact.
+__device__ __forceinline__ void bracket_factors(float beta, float& cb, float& c0) {
+  if (!(beta > 0.f)) {
+    cb = 0.f;
+    c0 = -INFINITY;
+    return;
+  }
+  const float inv_b = 1.0f / beta;
+  if (!(inv_b < 1e30f)) {
+    cb = 0.f;
+    c0 = INFINITY;
+    return;
+  }
+  cb = 0.69314718055994531f * inv_b;
+  c0 = 9.62e-7f * inv_b;
+}
+__device__ __forceinline__ void accept_bounds_f(uint32_t w, float cb, float c0, float& lo,
+                                                float& hi) {
+  // (w + 0.5) 2^-32 rounded as in accept_bounds (scaling by 2^-32 is exact)
+  const float uf = __builtin_fmaf((float)w, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
+  const float q = __builtin_amdgcn_logf(uf) * cb;
+  const float m = __builtin_fmaf(fabsf(q), 4.27e-6f, c0);
+  hi = q + m;
+  lo = q - m;
+}
+
 // The exact test (pysa/ising.py:52) for the rare dn inside the bracket.
 __device__ __noinline__ bool accept_exact(uint32_t w, float beta, float dn) {
   return dn >= 0.f || log_u32(w) < (double)beta * (double)dn;
@@ -946,8 +979,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   // Acceptance brackets of row block cr for sweep counter sw: this matrix
   // wave's share of the rows, lane = replica (beta from L.rb).
   auto brackets = [&](int cr, uint64_t sw, int pb) {
-    const float beta = L.rb[q];
-    const float inv_b = (beta > 0.f) ? 1.0f / beta : 0.f;
+    float cb, cz;
+    bracket_factors(L.rb[q], cb, cz);
     const int pos0 = 32 * cr;
     // groups of 4 rows (one Philox block each): one per matrix wave, the
     // rest to wave 4 (which shares a SIMD with the proposal wave only)
@@ -959,7 +992,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       for (int e = 0; e < 4; ++e) {
         const int r = 4 * g4 + e, pos = pos0 + r;
         float lo, hi;
-        accept_bounds(wd[e], beta, inv_b, lo, hi);
+        accept_bounds_f(wd[e], cb, cz, lo, hi);
         if (pos >= n || !act) lo = hi = INFINITY;  // padding and idle lanes: never taken
         L.lo[pb][r][q] = lo;
         L.hi[pb][r][q] = hi;
