@@ -724,23 +724,25 @@ __global__ __launch_bounds__(64 * KW) void blocked_pt_kernel(BParams p) {
 }
 
 // ------------------------------------------------------- pipelined form --
-// Four waves per group of 64 replicas.  Wave 0 runs the proposals with one
-// lane per replica: the lane holds all 32 fields of the block, so a decision
-// needs no cross-lane broadcast.  Waves 1-3 each own a third of the K range
-// and, while wave 0 sweeps block c, compute the next block's fields over
-// every K block except c (whose spins are changing):
+// Eight waves per group of 64 replicas, two per SIMD.  Wave 0 runs the
+// proposals with one lane per replica: the lane holds all 32 fields of the
+// block, so a decision needs no cross-lane broadcast.  Six matrix waves, a
+// third of the K range times a tile of 32 replicas each, compute the next
+// block's fields over every K block except c (whose spins are changing)
+// while wave 0 sweeps block c:
 //     P_{c+1} = sum_{b != c} J[c+1, b] . S_b
-// When wave 0 is done with block c, the owner of block c takes the new spins
-// and adds the missing product J[c+1, c] . S_c (one K block on the matrix
+// When wave 0 is done with block c, the owners of block c take the new spins
+// and add the missing product J[c+1, c] . S_c (one K block on the matrix
 // cores, from fragments kept back while computing P_{c+1}); the fields of
 // block c+1 are P_{c+1} plus that correction.  The matrix products of one
 // block therefore run under the proposals of the previous one.  The matrix
-// waves also prepare the next block's acceptance brackets, so the chain only
-// compares, flips and updates.  Shared state alternates between two buffers
-// by the parity of a running block counter.
-// Matrix waves of the pipelined kernel.  Four waves per workgroup keep one
-// wave per SIMD and its 512-VGPR budget; more waves would cap each at 256
-// registers, which the proposal wave's 32 fields and brackets exceed.
+// waves (and wave 4, which shares wave 0's SIMD) also prepare the next
+// block's acceptance brackets, so the chain only compares, flips and updates.
+// The two waves of a K range share a SIMD: gfx950 does not overlap a wave's
+// own VALU and MFMA instructions, but it does overlap different waves'.
+// Shared state alternates between two buffers by the parity of a running
+// block counter.
+// K ranges of the pipelined kernel (each served by two matrix waves).
 constexpr int kPipeMW = 3;
 constexpr int kPipePF = 3;  // fragment stages in flight per matrix wave (slot ring)
 
@@ -1377,7 +1379,7 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   const int groups = (reads + (64 / R) - 1) / (64 / R);
   int kw = 1;
   while (kw < 4 && 2 * kw <= nblk && groups * kw < 1024) kw *= 2;
-  // The pipelined kernel (proposal wave + three matrix waves) takes every
+  // The pipelined kernel (proposal wave + six matrix waves) takes every
   // problem of at least three blocks; PYSA_AMD_BLOCKED_SPLITK=1 selects the
   // split-K kernel instead (kept for small problems and comparisons).
   static const bool force_splitk = [] {
