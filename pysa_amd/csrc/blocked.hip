@@ -956,6 +956,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
 #pragma unroll kUnrollJ
     for (int jb0 = 0; jb0 < 2 * NW; jb0 += PF) {
       if (jb0 >= jn) break;
+      // stages left from this chunk on, recomputed per chunk (a scalar
+      // compare a slot instead of per-slot lane masks kept across the loop)
+      int jrem = jn - jb0;
+      asm volatile("" : "+s"(jrem));
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int j = jb0 + u;
@@ -963,7 +967,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         // fragments: the load is issued after the products that read them);
         // slots past the 2 NW spin halves exist only when PF does not divide
         // 2 NW and are never live
-        if (j < 2 * NW && j < jn) {
+        if (j < 2 * NW && u < jrem) {
           if (j == jx) {
 #pragma unroll
             for (int st = 0; st < NST; ++st)
