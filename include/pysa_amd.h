@@ -136,11 +136,13 @@ int pysa_amd_metropolis_device(int64_t handle, int32_t reads, int32_t R, int32_t
  *                pysa/simulation.py:305-316), 1 = before it (Neal 2001)
  *   states       in/out chains*n spins
  *   log_w        out chains log-weights (f64, without log Z0)
+ *   energies     out chains final energies as tracked (f64: recomputed every
+ *                64 steps, updated by the accepted changes in between), or NULL
  */
 int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, int32_t reference,
                  double p, int32_t order, int32_t ordering, uint64_t seed,
                  const double* replay_log_r, const int32_t* replay_perm, int8_t* states,
-                 double* log_w, double* kernel_ms);
+                 double* log_w, double* energies, double* kernel_ms);
 
 /*
  * Device-memory helpers for callers without their own HIP allocator (the
@@ -155,6 +157,15 @@ int pysa_amd_memcpy_d2d(void* dst, const void* src, uint64_t bytes);
 
 /* E = s.(J s / 2 + h) in f64 for `count` states (pysa/ising.py:32-37). */
 int pysa_amd_energy(int64_t handle, const int8_t* states, int64_t count, double* out);
+
+/*
+ * The plain form's resident local fields F = J s (without h) after the last
+ * plain-form run on this problem: the first `count` chains (read-major,
+ * count*n values in the problem dtype).  No counterpart in the reference,
+ * which recomputes every field from the coupling row; it measures the drift
+ * that the periodic f64 recomputation bounds.
+ */
+int pysa_amd_plain_fields(int64_t handle, int64_t count, void* out);
 
 /*
  * Many independent instances of different sizes in one launch.  PySA has no

@@ -36,8 +36,11 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# The same source built with -ffast-math: timed as the CPU baseline only.
+BASELINE_PATH = os.path.join(_HERE, "_build", "liboracle_baseline.so")
 
 _lib = None
+_baseline = None
 
 
 def build():
@@ -46,35 +49,47 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def load():
-    global _lib
+def load(baseline=False):
+    """The checker library, or with `baseline` its fast-math build (timing
+    only)."""
+    global _lib, _baseline
+    if baseline:
+        if _baseline is None:
+            if not os.path.exists(BASELINE_PATH):
+                build()
+            _baseline = _declare(ctypes.CDLL(BASELINE_PATH))
+        return _baseline
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = ctypes.CDLL(LIB_PATH)
-        vp, i32, u64, i64, dbl = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64,
-                                  ctypes.c_int64, ctypes.c_double)
-        lib.oracle_philox.argtypes = [vp, vp, vp]
-        lib.oracle_philox.restype = None
-        lib.oracle_det_log.argtypes = [vp, vp, i64]
-        lib.oracle_det_log.restype = None
-        lib.oracle_perm.argtypes = [ctypes.c_uint32, vp, vp]
-        lib.oracle_perm.restype = None
-        lib.oracle_sweep_stream.argtypes = [u64, u64, u64, i32, i32, vp, vp]
-        lib.oracle_sweep_stream.restype = None
-        lib.oracle_energy_f64.argtypes = [i32, vp, vp, vp, i64, vp]
-        lib.oracle_energy_f64.restype = None
-        lib.oracle_metropolis.argtypes = ([i32] * 3 + [vp, vp] + [i32] * 7 + [vp, vp, u64, u64] +
-                                          [vp] * 3 + [vp, vp, vp, vp, vp, vp, vp, vp, i32])
-        lib.oracle_metropolis.restype = i32
-        lib.oracle_ais.argtypes = ([i32] * 3 + [vp, vp, vp] + [i32] * 3 + [dbl, i32, i32, u64] +
-                                   [vp, vp, vp, vp, i32])
-        lib.oracle_ais.restype = i32
-        lib.oracle_batched.argtypes = ([i32] + [vp] * 7 + [i32] * 7 + [vp, u64, u64] + [vp] * 7 +
-                                       [i32])
-        lib.oracle_batched.restype = i32
-        _lib = lib
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def _declare(lib):
+    """Declare the C signatures on a loaded oracle library."""
+    vp, i32, u64, i64, dbl = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64,
+                              ctypes.c_int64, ctypes.c_double)
+    lib.oracle_philox.argtypes = [vp, vp, vp]
+    lib.oracle_philox.restype = None
+    lib.oracle_det_log.argtypes = [vp, vp, i64]
+    lib.oracle_det_log.restype = None
+    lib.oracle_perm.argtypes = [ctypes.c_uint32, vp, vp]
+    lib.oracle_perm.restype = None
+    lib.oracle_sweep_stream.argtypes = [u64, u64, u64, i32, i32, vp, vp]
+    lib.oracle_sweep_stream.restype = None
+    lib.oracle_energy_f64.argtypes = [i32, vp, vp, vp, i64, vp]
+    lib.oracle_energy_f64.restype = None
+    lib.oracle_metropolis.argtypes = ([i32] * 3 + [vp, vp] + [i32] * 7 + [vp, vp, u64, u64] +
+                                      [vp] * 3 + [vp, vp, vp, vp, vp, vp, vp, vp, i32])
+    lib.oracle_metropolis.restype = i32
+    lib.oracle_ais.argtypes = ([i32] * 3 + [vp, vp, vp] + [i32] * 3 + [dbl, i32, i32, u64] +
+                               [vp, vp, vp, vp, i32])
+    lib.oracle_ais.restype = i32
+    lib.oracle_batched.argtypes = ([i32] + [vp] * 7 + [i32] * 7 + [vp, u64, u64] + [vp] * 7 +
+                                   [i32])
+    lib.oracle_batched.restype = i32
+    return lib
 
 
 def _p(a):
@@ -118,9 +133,11 @@ def energy_f64(J, h, states):
 
 
 def metropolis(J, h, ptype, betas, states, energies, sweeps, *, order=0, use_pt=True,
-               log_omega=False, best_sem=0, seed=0, sweep_offset=0, replay=None, nthreads=1):
+               log_omega=False, best_sem=0, seed=0, sweep_offset=0, replay=None, nthreads=1,
+               baseline=False):
     """Run the oracle over all reads.  states: int8 (reads, R, n) (copied);
-    energies: (reads, R) in the float type of J.  Returns a dict of outputs."""
+    energies: (reads, R) in the float type of J.  Returns a dict of outputs.
+    `baseline` runs the fast-math build (for timing, not for checking)."""
     ft = np.dtype(J.dtype)
     J = np.ascontiguousarray(J)
     h = np.ascontiguousarray(h, dtype=ft)
@@ -142,7 +159,7 @@ def metropolis(J, h, ptype, betas, states, energies, sweeps, *, order=0, use_pt=
             pm = np.ascontiguousarray(replay["perm"], dtype=np.int32)
         if replay.get("pt_logu") is not None:
             pt = np.ascontiguousarray(replay["pt_logu"], dtype=np.float64)
-    rc = load().oracle_metropolis(n, ptype, 0 if ft == np.float32 else 1, _p(J), _p(h), reads, R,
+    rc = load(baseline).oracle_metropolis(n, ptype, 0 if ft == np.float32 else 1, _p(J), _p(h), reads, R,
                                   sweeps, order, int(use_pt), int(log_omega), best_sem, _p(betas),
                                   _p(bsort), seed & (2**64 - 1), sweep_offset, _p(lr), _p(pm),
                                   _p(pt), _p(st), _p(en), _p(bi), _p(bs), _p(be), _p(bw), _p(lw),

@@ -141,6 +141,36 @@ W4 ctr_perm(uint64_t chain, uint64_t sweep, uint32_t purpose) {
   return c;
 }
 
+// Block-random visit order (order 3), shared by every chain of a run: the
+// 32-spin blocks in the order of a Feistel permutation of [0, nblk) keyed by
+// the sweep (stream purpose 4), and inside block b the slots k = 0..31 visit
+// position 32 b + (k ^ m_b), m_b the low 5 bits of word b % 4 of the Philox
+// block (b / 4, sweep, 5 << 24, 0); positions >= n are skipped.
+struct BlockOrder {
+  int n, nblk;
+  uint64_t sweep;
+  uint32_t k0, k1;
+  Perm blocks;
+  BlockOrder(int n_, uint64_t sw, uint32_t k0_, uint32_t k1_)
+      : n(n_), nblk((n_ + 31) / 32), sweep(sw), k0(k0_), k1(k1_),
+        blocks((uint32_t)((n_ + 31) / 32), philox10(ctr_perm(0, sw, 4u), k0_, k1_)) {}
+  // Every position of the sweep in visit order.
+  std::vector<int> positions() const {
+    std::vector<int> out;
+    out.reserve(n);
+    for (int t = 0; t < nblk; ++t) {
+      const uint32_t b = blocks.at((uint32_t)t);
+      W4 c = {{b >> 2, (uint32_t)sweep, 5u << 24, 0u}};
+      const uint32_t m = philox10(c, k0, k1).v[b & 3] & 31u;
+      for (uint32_t k = 0; k < 32; ++k) {
+        const int pos = (int)(32 * b + (k ^ m));
+        if (pos < n) out.push_back(pos);
+      }
+    }
+    return out;
+  }
+};
+
 // ------------------------------------------------------------ problem -----
 // Coupling-row dot in the problem's float type, fixed order: 32 interleaved
 // partial sums (four independent 8-wide vector accumulators) combined
@@ -199,6 +229,8 @@ double sweep_one(const RunCfg& c, const T* J, const T* h, T* s, T beta, int read
   const uint64_t sw = (uint64_t)sweep + c.sweep_offset;
   const size_t base = (((size_t)read * c.S + sweep) * c.R + k) * (size_t)n;
   Perm perm(n, philox10(ctr_perm(c.order == 2 ? 0 : chain, sw, c.order == 2 ? 3u : 2u), k0, k1));
+  std::vector<int> border;
+  if (c.order == 3 && !c.rep_perm) border = BlockOrder(n, sw, k0, k1).positions();
   double de = 0.0;
   W4 blk = {{0, 0, 0, 0}};
   uint32_t blk_id = 0xFFFFFFFFu;
@@ -208,6 +240,8 @@ double sweep_one(const RunCfg& c, const T* J, const T* h, T* s, T beta, int read
       pos = c.rep_perm[base + i];
     else if (c.order == 0)
       pos = i;
+    else if (c.order == 3)
+      pos = border[i];
     else
       pos = (int)perm.at((uint32_t)i);
     double lr;
@@ -367,13 +401,17 @@ int ais(int n, int ptype, const T* J, const T* h, const T* temps, int nt, int ch
   for (int ch = 0; ch < chains; ++ch) {
     std::vector<T> s(n);
     for (int q = 0; q < n; ++q) s[q] = (T)states[(size_t)ch * n + q];
-    // energy in f64 and the count of "1" bits (ais.py:205-216)
-    double E = 0.0;
-    for (int i = 0; i < n; ++i) {
-      double r = 0.0;
-      for (int j = 0; j < n; ++j) r += (double)J[(size_t)i * n + j] * (double)s[j];
-      E += (double)s[i] * (0.5 * r + (double)h[i]);
-    }
+    // energy in f64 (ising.py:32-37) and the count of "1" bits (ais.py:205-216)
+    auto energy = [&]() {
+      double e = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < n; ++j) r += (double)J[(size_t)i * n + j] * (double)s[j];
+        e += (double)s[i] * (0.5 * r + (double)h[i]);
+      }
+      return e;
+    };
+    double E = energy();
     int ones = 0;
     for (int q = 0; q < n; ++q) ones += (s[q] == (T)1);
     double lw = 0.0, bold = 0.0;
@@ -388,11 +426,13 @@ int ais(int n, int ptype, const T* J, const T* h, const T* temps, int nt, int ch
       const uint64_t chain = (uint64_t)ch;
       const size_t base = ((size_t)ch * nt + t) * (size_t)n;
       Perm perm(n, philox10(ctr_perm(order == 2 ? 0 : chain, (uint64_t)t, order == 2 ? 3u : 2u), k0, k1));
+      std::vector<int> border;
+      if (order == 3 && !rep_perm) border = BlockOrder(n, (uint64_t)t, k0, k1).positions();
       W4 blk = {{0, 0, 0, 0}};
       uint32_t blk_id = 0xFFFFFFFFu;
-      double dsum = 0.0;  // f64 sum of the sweep's energy changes, added once
       for (int i = 0; i < n; ++i) {
-        int pos = rep_perm ? rep_perm[base + i] : (order == 0 ? i : (int)perm.at((uint32_t)i));
+        int pos = rep_perm ? rep_perm[base + i]
+                           : (order == 0 ? i : order == 3 ? border[i] : (int)perm.at((uint32_t)i));
         double lr;
         if (rep_lr)
           lr = rep_lr[base + pos];
@@ -411,10 +451,9 @@ int ais(int n, int ptype, const T* J, const T* h, const T* temps, int nt, int ch
         if (full >= 0 || lr < full) {
           ones += (s[pos] == (T)1) ? -1 : 1;
           s[pos] = flipped<T>(ptype, s[pos]);
-          dsum += -dn;
         }
       }
-      E += dsum;
+      E = energy();  // recomputed after every step (simulation.py:309)
       if (ordering == 0) {
         const double refE2 = (ref == 1) ? -((double)(n - ones) * lp + (double)ones * l1p) : 0.0;
         lw += (-(double)beta * E - (1.0 - (double)beta) * refE2) -
@@ -457,6 +496,11 @@ void oracle_sweep_stream(uint64_t seed, uint64_t chain, uint64_t sweep, int n, i
   for (int pos = 0; pos < n; ++pos) {
     W4 b = philox10(ctr_sweep(chain, sweep, (uint32_t)pos), k0, k1);
     log_r[pos] = dlog(u01(b.v[pos & 3]));
+  }
+  if (order == 3) {
+    const std::vector<int> b = BlockOrder(n, sweep, k0, k1).positions();
+    for (int i = 0; i < n; ++i) perm[i] = b[i];
+    return;
   }
   for (int i = 0; i < n; ++i) perm[i] = order == 0 ? i : (int32_t)p.at((uint32_t)i);
 }

@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpysa_amd.so")
 
 FORM_PLAIN, FORM_BLOCKED, FORM_AUTO, FORM_BLOCKED_EXACT = 0, 1, 2, 3
-ORDER = {"sequential": 0, "random": 1, "random_shared": 2}
+ORDER = {"sequential": 0, "random": 1, "random_shared": 2, "block_random": 3}
 
 _lib = None
 _lock = threading.Lock()
@@ -38,8 +38,9 @@ _SIGNATURES = {
                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                    ctypes.POINTER(_c_dbl), _vp],
     "pysa_amd_ais": [_c_i64, _vp, _c_i32, _c_i32, _c_i32, _c_dbl, _c_i32, _c_i32, _c_u64,
-                     _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
+                     _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
+    "pysa_amd_plain_fields": [_c_i64, _c_i64, _vp],
     "pysa_amd_batched_device": [_c_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32,
                                 _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _vp, _c_u64, _c_u64,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl),
@@ -132,6 +133,13 @@ class Problem:
         states = np.ascontiguousarray(states, dtype=np.int8).reshape(-1, self.n)
         out = np.empty(len(states), dtype=np.float64)
         check(load().pysa_amd_energy(self.handle, ptr(states), len(states), ptr(out)))
+        return out
+
+    def plain_fields(self, count):
+        """Local fields J s of the first `count` chains after the last
+        plain-form run (count x n, problem dtype)."""
+        out = np.empty((count, self.n), dtype=self.float_type)
+        check(load().pysa_amd_plain_fields(self.handle, count, ptr(out)))
         return out
 
 

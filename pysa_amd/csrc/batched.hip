@@ -142,25 +142,34 @@ __global__ __launch_bounds__(G >= 32 ? 512 : 1024) void batched_pt_kernel(BatchP
     const float beta = s_bs[r];
     const double bd = (double)beta;
     FeistelKey fk;
-    if (p.order != kSequential)
+    const bool brand = p.order == kBlockRandom;
+    const int nv = brand ? kOrderBlock * ((n + kOrderBlock - 1) / kOrderBlock) : n;
+    if (brand)
+      fk = block_order_key((uint32_t)((n + kOrderBlock - 1) / kOrderBlock), sw, p.k0, p.k1);
+    else if (p.order != kSequential)
       fk = feistel_make((uint32_t)n, perm_keys(p.order == kRandomShared ? 0ull : cid, sw,
                                                p.order == kRandomShared ? kStreamShared
                                                                         : kStreamPerm,
                                                p.k0, p.k1));
     double sum = 0.0;
-    for (int w = 0; w < n; w += 64) {
+    for (int w = 0; w < nv; w += 64) {
       const int vi = w + lane;
-      int pos_l = 0;
+      int pos_l = -1;
       double lr_l = 0.0;
-      if (vi < n) {
-        pos_l = (p.order == kSequential) ? vi : (int)feistel_perm(fk, (uint32_t)vi);
-        lr_l = log_u32(pick_word(sweep_block(cid, sw, (uint32_t)pos_l, p.k0, p.k1),
-                                 (uint32_t)pos_l & 3u));
+      if (vi < nv) {
+        if (brand)
+          pos_l = block_random_pos(fk, sw, (uint32_t)vi, n, p.k0, p.k1);
+        else
+          pos_l = (p.order == kSequential) ? vi : (int)feistel_perm(fk, (uint32_t)vi);
+        if (pos_l >= 0)
+          lr_l = log_u32(pick_word(sweep_block(cid, sw, (uint32_t)pos_l, p.k0, p.k1),
+                                   (uint32_t)pos_l & 3u));
       }
-      const float hl = (vi < n) ? s_h[pos_l] : 0.f;
-      const int cnt = min(64, n - w);
+      const float hl = (pos_l >= 0) ? s_h[pos_l] : 0.f;
+      const int cnt = min(64, nv - w);
       for (int t = 0; t < cnt; ++t) {
         const int pos = rl(pos_l, t);
+        if (pos < 0) continue;  // padding slot (wave-uniform)
         const double lr = rl(lr_l, t);
         const int pl = (pos >> 2) & 63, ps = ((pos >> 8) << 2) | (pos & 3);
         const bool up = (rl(ch.bits, pl) >> ps) & 1u;

@@ -46,6 +46,7 @@
 #include <vector>
 
 #include "../../include/pysa_amd.h"
+#include "accept.hpp"
 #include "engine.hpp"
 #include "rng.hpp"
 
@@ -75,13 +76,49 @@ __host__ __device__ inline float bf16_to_f(uint16_t b) {
   return f;
 }
 
-// Row sums of J in f64, rounded once (QUBO fields from +-1 products).
-__global__ void rowsum_kernel(const float* __restrict__ J, int n, float* __restrict__ out) {
+// Sum of the first np bf16 pieces of v (hi, then lo of the remainder, ...):
+// the coupling value the bf16 products multiply with.  Each remainder is
+// exact in f32 and the partial sums carry at most 8 np significant bits, so
+// the sum is exact as well.
+__host__ __device__ inline float bf16_pieces_sum(float v, int np) {
+  float acc = 0.f;
+  for (int p = 0; p < np; ++p) {
+    const float b = bf16_to_f(bf16_rn(v));
+    acc = acc + b;
+    v = v - b;
+  }
+  return acc;
+}
+
+// out = the pieces sum of every element of J (n*n).
+__global__ void pieces_sum_kernel(const float* __restrict__ J, long long count, int np,
+                                  float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = bf16_pieces_sum(J[i], np);
+}
+
+// Row sums of the couplings the products use (J, or its np-piece sums when
+// np > 0) in f64, rounded once (QUBO fields from +-1 products).
+__global__ void rowsum_kernel(const float* __restrict__ J, int n, int np, float* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int j = 0; j < n; ++j) s += (double)J[(size_t)i * n + j];
+  for (int j = 0; j < n; ++j) {
+    const float v = J[(size_t)i * n + j];
+    s += (double)(np > 0 ? bf16_pieces_sum(v, np) : v);
+  }
   out[i] = (float)s;
+}
+
+// x with bit k moved to bit k ^ m (m < 32, wave-uniform): the in-block
+// permutation of the block-random order, an involution.
+__device__ __forceinline__ uint32_t xor_permute_bits(uint32_t x, uint32_t m) {
+  if (m & 1u) x = ((x & 0x55555555u) << 1) | ((x >> 1) & 0x55555555u);
+  if (m & 2u) x = ((x & 0x33333333u) << 2) | ((x >> 2) & 0x33333333u);
+  if (m & 4u) x = ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
+  if (m & 8u) x = ((x & 0x00FF00FFu) << 8) | ((x >> 8) & 0x00FF00FFu);
+  if (m & 16u) x = (x << 16) | (x >> 16);
+  return x;
 }
 
 // Fragment (c, step, piece) of the bf16 form: lane l holds
@@ -173,72 +210,6 @@ __device__ __forceinline__ void spins_i8(uint32_t x, uint32_t mul, uint32_t xr, 
   }
 }
 
-// Certified acceptance thresholds.  PySA accepts a proposal iff
-// dn >= 0 or det_log(u) < beta*dn (product exact in f64), with
-// u = (w + 0.5) 2^-32.  For beta > 0 that is dn > q with q = det_log(u)/beta
-// (q < 0).  `accept_bounds` brackets q from the hardware logarithm:
-// dn > hi certainly accepts, dn < lo certainly rejects, and only a dn inside
-// [lo, hi] (a relative width of a few 1e-6) needs the exact test.  The bound
-// on |L - log(u)| covers the rounding of u to f32, the hardware log2 and the
-// scaling by ln 2; the bracket adds the rounding of q = L/beta and of hi, lo.
-// beta <= 0 accepts every proposal (hi = lo = -inf).
-__device__ __forceinline__ void accept_bounds(uint32_t w, float beta, float inv_b, float& lo,
-                                              float& hi) {
-  if (!(beta > 0.f)) {
-    lo = hi = -INFINITY;
-    return;
-  }
-  if (!(inv_b < 1e30f)) {  // beta too small to scale: every decision goes exact
-    lo = -INFINITY;
-    hi = INFINITY;
-    return;
-  }
-  const float uf = ((float)w + 0.5f) * 2.3283064365386963e-10f;
-  const float L = __builtin_amdgcn_logf(uf) * 0.69314718055994531f;
-  const float q = L * inv_b;
-  const float m = (3.9e-6f * fabsf(L) + 9.6e-7f) * inv_b * 1.001f + 3.6e-7f * fabsf(q);
-  hi = q + m;
-  lo = q - m;
-}
-
-// The same bracket with the per-replica factors taken out of the per-row
-// work: q = log2(uf) * cb with cb = ln2 / beta (one rounding fewer than
-// L = log2(uf) ln2 followed by q = L / beta, the same count overall), and
-// m = |q| A + c0 with A = 4.27e-6 >= 1.001 * 3.9e-6 (1 + 2^-22) + 3.6e-7,
-// c0 = 9.62e-7 / beta >= 1.001 * 9.6e-7 / beta, both rounded up so that the
-// fma's own rounding stays inside the margin of accept_bounds.  beta <= 0
-// gives cb = 0, c0 = -inf: hi = -inf, every proposal taken; beta too small
-// to scale gives cb = 0, c0 = +inf: every decision exact.
-__device__ __forceinline__ void bracket_factors(float beta, float& cb, float& c0) {
-  if (!(beta > 0.f)) {
-    cb = 0.f;
-    c0 = -INFINITY;
-    return;
-  }
-  const float inv_b = 1.0f / beta;
-  if (!(inv_b < 1e30f)) {
-    cb = 0.f;
-    c0 = INFINITY;
-    return;
-  }
-  cb = 0.69314718055994531f * inv_b;
-  c0 = 9.62e-7f * inv_b;
-}
-__device__ __forceinline__ void accept_bounds_f(uint32_t w, float cb, float c0, float& lo,
-                                                float& hi) {
-  // (w + 0.5) 2^-32 rounded as in accept_bounds (scaling by 2^-32 is exact)
-  const float uf = __builtin_fmaf((float)w, 2.3283064365386963e-10f, 1.1641532182693481e-10f);
-  const float q = __builtin_amdgcn_logf(uf) * cb;
-  const float m = __builtin_fmaf(fabsf(q), 4.27e-6f, c0);
-  hi = q + m;
-  lo = q - m;
-}
-
-// The exact test (pysa/ising.py:52) for the rare dn inside the bracket.
-__device__ __noinline__ bool accept_exact(uint32_t w, float beta, float dn) {
-  return dn >= 0.f || log_u32(w) < (double)beta * (double)dn;
-}
-
 // f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}):
 // a loop whose index is a compile-time constant in the body, so arrays indexed
 // by expressions of it stay in registers.
@@ -286,11 +257,16 @@ __device__ __forceinline__ void set_half(Bits<NW>& s, int c, uint32_t v) {
 
 struct BParams {
   int n, nblk, ptype, R, reads, S, rpw, np, use_pt, logw, best_sem;
+  int order;  // kSequential or kBlockRandom (pipelined kernel only)
   int kb;  // row blocks of the K range each wave of a group owns
   uint32_t k0, k1;
   uint64_t sweep_offset;
   const void* frag;
-  const float* Jt;  // f32 J^T, row j = coefficients J[:][j] (in-block updates)
+  // f32 J^T with every coupling replaced by the sum of its bf16 pieces (J
+  // itself when the pieces are exact), row j = coefficients J[:][j]: the
+  // in-block updates use the same couplings as the matrix products, so every
+  // decision and energy change comes from one energy function
+  const float* Jt;
   const float* h;
   const float* rowsum;  // n row sums of J (QUBO with bf16 operands), else null
   const float* betas;
@@ -763,6 +739,9 @@ struct __align__(16) PipeLds {
   uint32_t bb[2][64];         // block spins before the block (bit r = row r)
   uint32_t bnew[64];          // block spins after the block
   float sfp[64 * 33];         // exact rerun staging
+  // visit schedule by sweep parity: block at step t, in-block XOR mask of block b
+  uint8_t ord[2][64];
+  uint8_t msk[2][64];
   float E[64];
   float rb[64];
   int bi[64];
@@ -867,9 +846,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
 #pragma unroll
       for (int pc = 0; pc < NP; ++pc) dst[st][pc] = base[(st * NP + pc) * 64 + lane];
   };
-  // Request row block c: the first PF fragment stages of this wave's K
-  // range and (spread over the matrix lanes) J[blk][blk], h and row sums.
-  auto prefetch_block = [&](int c) {
+  // Request row block c, visited with in-block mask xm: the first PF
+  // fragment stages of this wave's K range and (spread over the matrix
+  // lanes) J[blk][blk] in visit order, h and row sums.
+  auto prefetch_block = [&](int c, uint32_t xm) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) load_stage(c, u, ring[u]);
     const int pos0 = 32 * c;
@@ -880,10 +860,13 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
       const int e = mt + e2 * 128 * MW;
-      const int i = e >> 5, k = i + 1 + (e & 31);  // slot t = e & 31 holds column i + 1 + t
-      const bool ok = e < kB * kB && k < kB && pos0 + i < n && pos0 + k < n;
+      // slot t = e & 31 of visit row i holds visit column k = i + 1 + t;
+      // visit index v is block row v ^ xm
+      const int i = e >> 5, k = i + 1 + (e & 31);
+      const int ri = i ^ (int)xm, rk = k ^ (int)xm;
+      const bool ok = e < kB * kB && k < kB && pos0 + ri < n && pos0 + rk < n;
       jb_ok |= (uint32_t)ok << e2;
-      jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + i) * n + pos0 + k : 0];
+      jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
     }
     const bool hok = mt >= 0 && mt < kB && pos0 + mt < n;
     jb_ok |= (uint32_t)hok << 31;
@@ -982,9 +965,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     }
     write_fields(dst, acc, iacc);
   };
-  // Acceptance brackets of row block cr for sweep counter sw: this matrix
-  // wave's share of the rows, lane = replica (beta from L.rb).
-  auto brackets = [&](int cr, uint64_t sw, int pb) {
+  // Acceptance brackets of row block cr (in-block mask xm) for sweep counter
+  // sw, stored by visit index: this matrix wave's share of the rows, lane =
+  // replica (beta from L.rb).
+  auto brackets = [&](int cr, uint32_t xm, uint64_t sw, int pb) {
     float cb, cz;
     bracket_factors(L.rb[q], cb, cz);
     const int pos0 = 32 * cr;
@@ -1000,8 +984,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         float lo, hi;
         accept_bounds_f(wd[e], cb, cz, lo, hi);
         if (pos >= n || !act) lo = hi = INFINITY;  // padding and idle lanes: never taken
-        L.lo[pb][r][q] = lo;
-        L.hi[pb][r][q] = hi;
+        L.lo[pb][r ^ (int)xm][q] = lo;
+        L.hi[pb][r ^ (int)xm][q] = hi;
       }
     }
   };
@@ -1013,17 +997,44 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     if (lane < 32) L.bb[pb][qt] = block_of<I8>(mine, got);
   };
 
-  // ---- prologue: the fields of block 0 in full, its brackets and spins
-  __syncthreads();  // wave 0's betas (L.rb) and the zeroed L.corr are visible
+  // Visit schedule of sweep counter swn into parity par (wave 4, one lane
+  // per block): the block visited at step b and the in-block mask of block b.
+  auto fill_sched = [&](int par, uint64_t swn) {
+    if (lane < nblk) {
+      int bo = lane;
+      uint32_t mk = 0;
+      if (p.order == kBlockRandom) {
+        const FeistelKey fk = block_order_key((uint32_t)nblk, swn, p.k0, p.k1);
+        bo = (int)feistel_perm(fk, (uint32_t)lane);
+        mk = block_mask(swn, (uint32_t)lane, p.k0, p.k1);
+      }
+      L.ord[par][lane] = (uint8_t)bo;
+      L.msk[par][lane] = (uint8_t)mk;
+    }
+  };
+  auto sched_block = [&](int par, int step) {
+    return __builtin_amdgcn_readfirstlane((int)L.ord[par][step]);
+  };
+  auto sched_mask = [&](int par, int c) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)L.msk[par][c]);
+  };
+
+  // ---- prologue: the fields of the first block in full, its brackets and spins
+  if (w == 4) fill_sched(0, p.sweep_offset);
+  __syncthreads();  // wave 0's betas (L.rb), the zeroed L.corr and the schedule are visible
   const uint64_t sw0 = p.sweep_offset;
-  if (mw) {
-    prefetch_block(0);
-    publish_block(0);
-    compute_part(0, -1, L.part[0][m]);
-    prefetch_block(nblk > 1 ? 1 : 0);
-    if (wl == owner_of(0)) publish_bits(0, 0);
+  {
+    const int cf = sched_block(0, 0), cs = sched_block(0, 1);
+    const uint32_t mf = sched_mask(0, cf), ms = sched_mask(0, cs);
+    if (mw) {
+      prefetch_block(cf, mf);
+      publish_block(0);
+      compute_part(cf, -1, L.part[0][m]);
+      prefetch_block(cs, ms);
+      if (wl == owner_of(cf)) publish_bits(cf, 0);
+    }
+    if (mw || w == 4) brackets(cf, mf, sw0, 0);
   }
-  if (mw || w == 4) brackets(0, sw0, 0);
   __syncthreads();
 
   const float cu = (p.ptype == 0) ? 2.f : 1.f;  // |dn / (F + h)|
@@ -1039,18 +1050,25 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   if (w == 0) {
     for (int s = 0; s < p.S; ++s) {
       const uint64_t sw = (uint64_t)s + p.sweep_offset;
+      const int sp = s & 1;
       const float beta = act ? L.rb[q] : 0.f;
       double dsum = 0.0;
       uint32_t nacc = 0;
-      for (int c = 0; c < nblk; ++c, ++t) {
+      for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1;
+        // block c of this step, its visit k being block row k ^ xm
+        const int c = sched_block(sp, tb);
+        const uint32_t xm = sched_mask(sp, c);
         const int pos0 = 32 * c;
         // ---- proposals of block c, one lane per replica
         float fp[kB];
         {
+          const int xg = (int)(xm >> 2);
 #pragma unroll
           for (int r4 = 0; r4 < kB / 4; ++r4) {
-            const int o = fld(q, 4 * r4);
+            // visit group r4 holds block rows 4 (r4 ^ xg) + (e ^ (xm & 3))
+            const int gr = r4 ^ xg;
+            const int o = fld(q, 4 * gr);
             float4 sm = *reinterpret_cast<const float4*>(&L.part[pb][0][o]);
 #pragma unroll
             for (int mm = 1; mm < MW; ++mm) {
@@ -1061,16 +1079,34 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               sm.w = sm.w + a.w;
             }
             const float4 cr = *reinterpret_cast<const float4*>(&L.corr[o]);
-            const float4 h4 = *reinterpret_cast<const float4*>(&L.hb[pb][4 * r4]);
-            const float4 s4 = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * r4]);
+            const float4 h4 = *reinterpret_cast<const float4*>(&L.hb[pb][4 * gr]);
+            const float4 s4 = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
             const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y, sm.z + cr.z + s4.z,
                                  sm.w + cr.w + s4.w};
             const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
+            float f4[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) fp[4 * r4 + e] = __builtin_fmaf(fsc, vs[e], hv[e]);
+            for (int e = 0; e < 4; ++e) f4[e] = __builtin_fmaf(fsc, vs[e], hv[e]);
+            if (xm & 1u) {
+              const float a0 = f4[0], a2 = f4[2];
+              f4[0] = f4[1];
+              f4[1] = a0;
+              f4[2] = f4[3];
+              f4[3] = a2;
+            }
+            if (xm & 2u) {
+              const float a0 = f4[0], a1 = f4[1];
+              f4[0] = f4[2];
+              f4[1] = f4[3];
+              f4[2] = a0;
+              f4[3] = a1;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) fp[4 * r4 + e] = f4[e];
           }
         }
-        uint32_t mb = L.bb[pb][q];
+        // spins in visit order (bit k = block row k ^ xm)
+        uint32_t mb = xor_permute_bits(L.bb[pb][q], xm);
         const uint32_t mb0 = mb;
         // bit k of mb is only changed at step k, so every step reads its
         // spin from the block's initial bits: sign bit = !spin, magnitude cu
@@ -1125,11 +1161,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // exact test for the decisions inside their brackets (rolled loop,
           // fields staged in LDS).
           float* sf = &L.sfp[q * 33];
-          for (int r = 0; r < kB; ++r) {
-            float v = L.part[pb][0][fld(q, r)];
-            for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(q, r)];
-            v = v + L.corr[fld(q, r)];
-            sf[r] = __builtin_fmaf(fsc, v + L.rsb[pb][r], L.hb[pb][r]);
+          for (int r = 0; r < kB; ++r) {  // visit order, as fp
+            const int br = r ^ (int)xm;
+            float v = L.part[pb][0][fld(q, br)];
+            for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(q, br)];
+            v = v + L.corr[fld(q, br)];
+            sf[r] = __builtin_fmaf(fsc, v + L.rsb[pb][br], L.hb[pb][br]);
           }
           mb = mb0;
           dpart = 0.f;
@@ -1140,8 +1177,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             // idle lanes and padding rows have lo = hi = +inf: never taken
             bool take = act & (dn > L.hi[pb][k][q]);
             if (act && !take && !(dn < L.lo[pb][k][q])) {
-              const uint32_t wd = pick_word(sweep_block(cid, sw, (uint32_t)(pos0 + k), p.k0, p.k1),
-                                            (uint32_t)(pos0 + k) & 3u);
+              const uint32_t pk = (uint32_t)(pos0 + (k ^ (int)xm));
+              const uint32_t wd = pick_word(sweep_block(cid, sw, pk, p.k0, p.k1), pk & 3u);
               take = accept_exact(wd, beta, dn);
             }
             mb ^= (uint32_t)take << k;
@@ -1151,11 +1188,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               sf[r] = __builtin_fmaf(L.jb[pb][k * kB + (r - k - 1)], dS, sf[r]);
           }
         }
-        L.bnew[q] = mb;
+        L.bnew[q] = xor_permute_bits(mb, xm);  // back to block-row order
         dsum += (double)dpart;
         nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
         __syncthreads();  // B2
-        if (c + 1 == nblk) {
+        if (tb + 1 == nblk) {
           // ---- end of sweep: energy (simulation.py:157-158), best states, PT
           acc += nacc;
           E = (float)((double)E + dsum);
@@ -1224,18 +1261,26 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   } else {
     for (int s = 0; s < p.S; ++s) {
       const uint64_t sw = (uint64_t)s + p.sweep_offset;
-      for (int c = 0; c < nblk; ++c, ++t) {
+      const int sp = s & 1;
+      for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1, pn = pb ^ 1;
-        const int c1 = (c + 1 == nblk) ? 0 : c + 1;
         (void)pb;
+        // the schedule of the next sweep, needed from step nblk - 2 on
+        if (w == 4 && tb == 0) fill_sched(sp ^ 1, sw + 1);
+        // block c of this step; c1 and c2 the next two (into the next sweep)
+        const bool last = tb + 1 == nblk;
+        const int c = sched_block(sp, tb);
+        const int p1 = last ? sp ^ 1 : sp, p2 = tb + 2 < nblk ? sp : sp ^ 1;
+        const int c1 = sched_block(p1, last ? 0 : tb + 1);
+        const int c2 = sched_block(p2, tb + 2 < nblk ? tb + 2 : tb + 2 - nblk);
+        const uint32_t m1 = sched_mask(p1, c1);
         // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
         if (mw) {
           publish_block(pn);
-          const int c2 = c1 + 1 < nblk ? c1 + 1 : 0;
           compute_part(c1, c, L.part[pn][m]);
-          prefetch_block(c2);
+          prefetch_block(c2, sched_mask(p2, c2));
         }
-        if ((mw || w == 4) && c1 != 0) brackets(c1, sw, pn);
+        if ((mw || w == 4) && !last) brackets(c1, m1, sw, pn);
         __syncthreads();  // B2
         if (mw && wl == owner_of(c)) {
           // ---- the owners of block c take its new spins and add J[c1, c] . S_c
@@ -1247,7 +1292,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           write_fields(L.corr, acc, iacc);
         }
         if (mw && wl == owner_of(c1)) publish_bits(c1, pn);
-        if (c + 1 == nblk) {
+        if (last) {
           __syncthreads();
           if (mw && p.best_sem == 0 && actt && L.flag[qt])
             store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_t);
@@ -1255,7 +1300,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // owners copy their part of a new global best
           if (mw && p.best_sem == 1 && actt && L.src[gt] == qt - gt * R)
             store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gt) * n);
-          if (mw || w == 4) brackets(0, sw + 1, pn);  // next sweep's first block, new betas
+          if (mw || w == 4) brackets(c1, m1, sw + 1, pn);  // next sweep's first block, new betas
         }
         __syncthreads();  // B3
       }
@@ -1338,12 +1383,15 @@ void blocked_release(BlockedProblem& b) {
   if (b.packed) (void)hipFree(b.packed);
   if (b.scratch) (void)hipFree(b.scratch);
   if (b.rowsum) (void)hipFree(b.rowsum);
+  if (b.jtl) (void)hipFree(b.jtl);
   b = BlockedProblem();
 }
 
 bool blocked_eligible(int n, int dtype, int R, int reads, int order, bool replay) {
+  // the block-random order runs on the pipelined kernel (at least 3 blocks)
+  const bool ord_ok = order == kSequential || (order == kBlockRandom && n > 2 * kB);
   return dtype == 0 && n >= 32 && n <= kBlockedMaxN && R >= 1 && R <= 64 && reads >= 1 &&
-         order == kSequential && !replay;
+         ord_ok && !replay;
 }
 
 // Smallest exact coupling encoding: 0 = int8, 1..3 = bf16 pieces; 4 = none
@@ -1373,8 +1421,8 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
                 double* log_omegas, uint64_t* accepted, hipStream_t st, double* ms,
                 std::string* why) {
   if (!blocked_eligible(n, dtype, R, reads, order, rep_lr != nullptr || rep_pt != nullptr)) {
-    *why = "blocked form needs float32 couplings, 32 <= n <= 2048, R <= 64, sequential order "
-           "and Philox randomness (no replay)";
+    *why = "blocked form needs float32 couplings, 32 <= n <= 2048, R <= 64, sequential or "
+           "block-random order (the latter for n > 64) and Philox randomness (no replay)";
     return PYSA_AMD_ENOTSUP;
   }
   const int nblk = (n + kB - 1) / kB;
@@ -1390,7 +1438,7 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     const char* e = getenv("PYSA_AMD_BLOCKED_SPLITK");
     return e && e[0] == '1';
   }();
-  const bool pipe = nblk >= kPipeMW && !force_splitk;
+  const bool pipe = nblk >= kPipeMW && (!force_splitk || order == kBlockRandom);
   const int kbp = (nblk + kPipeMW - 1) / kPipeMW;
   int nwp = 4;
   while (2 * nwp < kbp) nwp *= 2;
@@ -1439,18 +1487,38 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     }
     b.packed_kind = kind;
   }
-  if (!i8 && ptype == 1 && !b.rowsum) {
-    if (hipMalloc(&b.rowsum, sizeof(float) * n) != hipSuccess) {
+  // The pieces do not hold every coupling exactly: the in-block updates and
+  // the QUBO row sums then use the piece sums too (the couplings the products
+  // multiply with), so all fields derive from one coupling matrix.
+  const bool inexact = !i8 && kind < b.exact_kind;
+  if (inexact && b.jtl_kind != kind) {
+    if (!b.jtl && hipMalloc(&b.jtl, sizeof(float) * (size_t)n * n) != hipSuccess) {
+      b.jtl = nullptr;
+      *why = "allocating piece-sum couplings";
+      return PYSA_AMD_EHIP;
+    }
+    const long long cnt2 = (long long)n * n;
+    hipLaunchKernelGGL(pieces_sum_kernel, dim3((unsigned)((cnt2 + 255) / 256)), dim3(256), 0, st,
+                       (const float*)dJt, cnt2, np, b.jtl);
+    if (hipGetLastError() != hipSuccess) {
+      *why = "piece-sum couplings";
+      return PYSA_AMD_EHIP;
+    }
+    b.jtl_kind = kind;
+  }
+  if (!i8 && ptype == 1 && b.rowsum_kind != kind) {
+    if (!b.rowsum && hipMalloc(&b.rowsum, sizeof(float) * n) != hipSuccess) {
       b.rowsum = nullptr;
       *why = "allocating row sums";
       return PYSA_AMD_EHIP;
     }
     hipLaunchKernelGGL(rowsum_kernel, dim3((n + 255) / 256), dim3(256), 0, st, (const float*)dJ, n,
-                       b.rowsum);
+                       inexact ? np : 0, b.rowsum);
     if (hipGetLastError() != hipSuccess) {
       *why = "row sums";
       return PYSA_AMD_EHIP;
     }
+    b.rowsum_kind = kind;
   }
   const size_t cnt = (size_t)reads * R;
   const size_t need = cnt * n + cnt * 4 + cnt * 4;
@@ -1480,8 +1548,9 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   p.k1 = (uint32_t)(seed >> 32);
   p.sweep_offset = sweep_offset;
   p.frag = b.packed;
-  p.Jt = (const float*)dJt;
+  p.Jt = inexact ? b.jtl : (const float*)dJt;
   p.h = (const float*)dh;
+  p.order = order;
   p.rowsum = (!i8 && ptype == 1) ? b.rowsum : nullptr;
   p.betas = (const float*)betas;
   p.bsort = bsort;

@@ -20,11 +20,15 @@ struct BlockedProblem {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
   float* rowsum = nullptr;  // J . 1 per row (QUBO with bf16 operands)
+  int rowsum_kind = -1;     // encoding the row sums were taken for
+  float* jtl = nullptr;     // J^T as its bf16 piece sums (inexact encodings only)
+  int jtl_kind = -1;
 };
 
 void blocked_release(BlockedProblem& b);
 
-// Whether the blocked form can run this request.
+// Whether the blocked form can run this request (sequential order, or the
+// block-random order for n > 64).
 bool blocked_eligible(int n, int dtype, int R, int reads, int order, bool replay);
 
 // pieces: 0 = exact where possible else hi+lo bf16; 2 or 3 = that many bf16

@@ -134,7 +134,7 @@ int check_run(const Problem* P, const RunArgs& a) {
     return fail(PYSA_AMD_EINVAL, "null array argument");
   if (a.logw && (!a.bsort || !a.log_omegas))
     return fail(PYSA_AMD_EINVAL, "log_omega needs beta_sort and log_omegas");
-  if (a.order < 0 || a.order > 2) return fail(PYSA_AMD_EINVAL, "order must be 0, 1 or 2");
+  if (a.order < 0 || a.order > 3) return fail(PYSA_AMD_EINVAL, "order must be 0, 1, 2 or 3");
   if (a.best_sem < 0 || a.best_sem > 1) return fail(PYSA_AMD_EINVAL, "best_semantics must be 0 or 1");
   if (a.form == PYSA_AMD_FORM_PLAIN && P->n > pysa::kPlainMaxN)
     return fail(PYSA_AMD_ENOTSUP, "n above the plain kernel's register budget (4096)");
@@ -158,7 +158,7 @@ int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
   p.use_pt = a.use_pt;
   p.logw = a.logw;
   p.best_sem = a.best_sem;
-  p.refresh = 64;
+  p.refresh = 32;
   p.k0 = (uint32_t)a.seed;
   p.k1 = (uint32_t)(a.seed >> 32);
   p.sweep_offset = a.sweep_offset;
@@ -441,7 +441,7 @@ int pysa_amd_metropolis(int64_t handle, int32_t reads, int32_t R, int32_t S, int
 int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, int32_t reference,
                  double p, int32_t order, int32_t ordering, uint64_t seed,
                  const double* replay_log_r, const int32_t* replay_perm, int8_t* states,
-                 double* log_w, double* kernel_ms) {
+                 double* log_w, double* energies, double* kernel_ms) {
   g_err.clear();
   Problem* P = get(handle);
   if (!P) return fail(PYSA_AMD_EINVAL, "invalid problem handle");
@@ -449,7 +449,7 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   if (!temps || !states || !log_w) return fail(PYSA_AMD_EINVAL, "null array argument");
   if (reference != 0 && reference != 1) return fail(PYSA_AMD_EINVAL, "reference must be 0 or 1");
   if (reference == 1 && !(p > 0.0 && p < 1.0)) return fail(PYSA_AMD_EINVAL, "Bernoulli p must be in (0, 1)");
-  if (order < 0 || order > 2) return fail(PYSA_AMD_EINVAL, "order must be 0, 1 or 2");
+  if (order < 0 || order > 3) return fail(PYSA_AMD_EINVAL, "order must be 0, 1, 2 or 3");
   if (P->n > pysa::kPlainMaxN) return fail(PYSA_AMD_ENOTSUP, "n above 4096");
   std::lock_guard<std::mutex> lk(P->mu);
   HIP_TRY(hipSetDevice(P->device));
@@ -473,7 +473,7 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   hipError_t le;
   const double lp = reference == 1 ? log(p) : 0.0, l1p = reference == 1 ? log(1.0 - p) : 0.0;
   if (P->dtype == 0) {
-    pysa::AISParams<float> a = {(int)n, P->ptype, chains, nt, order, reference, ordering,
+    pysa::AISParams<float> a = {(int)n, P->ptype, chains, nt, order, reference, ordering, 64,
                                 (uint32_t)seed, (uint32_t)(seed >> 32), lp, l1p,
                                 (const float*)P->dJt, (const float*)P->dJp, P->ldp,
                                 (const float*)P->dh, (const float*)dT,
@@ -481,7 +481,7 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
                                 (double*)dE, (double*)dW};
     le = pysa::launch_plain_ais<float>(a, st);
   } else {
-    pysa::AISParams<double> a = {(int)n, P->ptype, chains, nt, order, reference, ordering,
+    pysa::AISParams<double> a = {(int)n, P->ptype, chains, nt, order, reference, ordering, 64,
                                  (uint32_t)seed, (uint32_t)(seed >> 32), lp, l1p,
                                  (const double*)P->dJt, (const double*)P->dJp, P->ldp,
                                  (const double*)P->dh, (const double*)dT,
@@ -499,6 +499,8 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   if (kernel_ms) *kernel_ms = t;
   HIP_TRY(hipMemcpy(states, dS, (size_t)chains * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(log_w, dW, (size_t)chains * sizeof(double), hipMemcpyDeviceToHost));
+  if (energies)
+    HIP_TRY(hipMemcpy(energies, dE, (size_t)chains * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -530,6 +532,20 @@ int pysa_amd_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
 int pysa_amd_memcpy_d2d(void* dst, const void* src, uint64_t bytes) {
   g_err.clear();
   HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return 0;
+}
+
+int pysa_amd_plain_fields(int64_t handle, int64_t count, void* out) {
+  g_err.clear();
+  Problem* P = get(handle);
+  if (!P) return fail(PYSA_AMD_EINVAL, "invalid problem handle");
+  if (count < 0 || (count > 0 && !out)) return fail(PYSA_AMD_EINVAL, "bad arguments");
+  std::lock_guard<std::mutex> lk(P->mu);
+  const size_t bytes = (size_t)count * P->n * esize(P->dtype);
+  if (bytes > P->fields.bytes) return fail(PYSA_AMD_EINVAL, "more fields than the last plain run kept");
+  if (count == 0) return 0;
+  HIP_TRY(hipSetDevice(P->device));
+  HIP_TRY(hipMemcpy(out, P->fields.p, bytes, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -570,9 +586,9 @@ extern "C" int pysa_amd_batched_device(int32_t count, const int32_t* n, const in
                                        int32_t* best_sweeps, uint64_t* accepted, double* kernel_ms,
                                        void* stream) {
   g_err.clear();
-  if (count < 1 || reads < 1 || R < 1 || R > 16 || S < 1 || order < 0 || order > 2 ||
+  if (count < 1 || reads < 1 || R < 1 || R > 16 || S < 1 || order < 0 || order > 3 ||
       problem_type < 0 || problem_type > 1 || best_semantics < 0 || best_semantics > 1)
-    return fail(PYSA_AMD_EINVAL, "batched: count, reads >= 1, 1 <= R <= 16, S >= 1, order 0-2");
+    return fail(PYSA_AMD_EINVAL, "batched: count, reads >= 1, 1 <= R <= 16, S >= 1, order 0-3");
   if (!n || !ld || !j_off || !signs || !scale || !betas || !s_off || !states || !energies ||
       !beta_idx || !best_state || !best_energy || !best_sweeps || (h && !h_off))
     return fail(PYSA_AMD_EINVAL, "batched: null array");

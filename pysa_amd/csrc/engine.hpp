@@ -5,7 +5,10 @@
 
 namespace pysa {
 
-enum Order : int { kSequential = 0, kRandom = 1, kRandomShared = 2 };
+// Visit orders: sequential; a fresh Feistel permutation per chain and sweep;
+// one permutation per sweep shared by every chain; the block-random order
+// (rng.hpp) shared by every chain, the random order the MFMA form runs.
+enum Order : int { kSequential = 0, kRandom = 1, kRandomShared = 2, kBlockRandom = 3 };
 
 // Device-side parameters of one Metropolis / parallel-tempering run over
 // `reads` independent reads of R replicas of one problem (plain form).
@@ -43,6 +46,7 @@ struct PTParams {
 template <typename T>
 struct AISParams {
   int n, ptype, chains, nt, order, ref, ordering;
+  int refresh;  // recompute fields and energy in f64 every `refresh` steps (0 = never)
   uint32_t k0, k1;
   double lp, l1p;  // log p, log(1-p) of the Bernoulli reference
   const T* Jt;

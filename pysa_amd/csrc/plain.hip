@@ -262,27 +262,36 @@ __device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const 
   const int n = c.n, lane = threadIdx.x & 63;
   od = 0;
   if (!active) return 0.0;
+  // Visit slots: n, or 32 per block in the block-random order (padding
+  // slots of the last block visit nothing).
+  const bool brand = c.order == kBlockRandom && rep_perm == nullptr;
+  const int nv = brand ? kOrderBlock * ((n + kOrderBlock - 1) / kOrderBlock) : n;
   FeistelKey fk;
-  if (c.order != kSequential && rep_perm == nullptr)
+  if (brand)
+    fk = block_order_key((uint32_t)((n + kOrderBlock - 1) / kOrderBlock), c.sw, c.k0, c.k1);
+  else if (c.order != kSequential && rep_perm == nullptr)
     fk = feistel_make((uint32_t)n,
                       perm_keys(c.order == kRandomShared ? 0ull : cid, c.sw,
                                 c.order == kRandomShared ? kStreamShared : kStreamPerm, c.k0, c.k1));
   double sum = 0.0;
-  for (int w = 0; w < n; w += 64) {
+  for (int w = 0; w < nv; w += 64) {
     const int vi = w + lane;
-    int pos_l = 0;
+    int pos_l = -1;
     double lr_l = 0.0;
-    if (vi < n) {
+    if (vi < nv) {
       if (rep_perm)
         pos_l = rep_perm[vi];
+      else if (brand)
+        pos_l = block_random_pos(fk, c.sw, (uint32_t)vi, n, c.k0, c.k1);
       else
         pos_l = (c.order == kSequential) ? vi : (int)feistel_perm(fk, (uint32_t)vi);
-      lr_l = log_uniform(rep_lr, cid, c.sw, pos_l, c.k0, c.k1);
+      if (pos_l >= 0) lr_l = log_uniform(rep_lr, cid, c.sw, pos_l, c.k0, c.k1);
     }
-    const T h_l = (vi < n) ? hs[pos_l] : T(0);
-    const int cnt = min(64, n - w);
+    const T h_l = (pos_l >= 0) ? hs[pos_l] : T(0);
+    const int cnt = min(64, nv - w);
     for (int t = 0; t < cnt; ++t) {
       const int pos = rl(pos_l, t);
+      if (pos < 0) continue;  // padding slot (wave-uniform)
       const double lr = rl(lr_l, t);
       const int pl = lane_of(pos), ps = slot_of(pos);
       const bool up = (rl(ch.bits, pl) >> ps) & 1;
@@ -312,6 +321,7 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(P
   __shared__ T s_E[kMaxR];
   __shared__ long long s_bi[kMaxR];
   __shared__ double s_ptlu[kMaxR];
+  __shared__ double s_eoff[kMaxR];  // caller's initial energy minus its f64 recomputation
   __shared__ int s_src;
   __shared__ T s_gE;
   __shared__ int s_gsw, s_improved;
@@ -340,14 +350,27 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(P
   for (int s = 0; s < p.S; ++s) {
     c.sw = (uint64_t)s + p.sweep_offset;
     // Local fields F = J s recomputed in f64 (rounded once) at the start and
-    // every `refresh` sweeps, which bounds the drift of float couplings.
+    // every `refresh` sweeps, which bounds the drift of float couplings.  The
+    // reference recomputes every field from its coupling row
+    // (ising.py:48), so its energy changes carry no drift; here the tracked
+    // energy is reset with the fields to the f64 energy plus the offset the
+    // caller's initial energy had from it (zero unless the caller passed
+    // other energies), so the drift of incrementally updated fields does not
+    // accumulate.  Integer couplings are exact either way: the reset then
+    // leaves the value unchanged.
     if (s == 0 || (p.refresh > 0 && s % p.refresh == 0)) {
       for (int r = wave; r < R; r += W) {
         Chain<T, G> ch;
         const size_t off = ((size_t)read * R + r) * n;
         ch.load(p.fields + off, p.spins + off, n);
-        ch.refresh(p.Jt, s_h, n, p.ptype);
+        const double ex = ch.refresh(p.Jt, s_h, n, p.ptype);
         ch.store(p.fields + off, p.spins + off, n, p.ptype);
+        if (lane == 0) {
+          if (s == 0)
+            s_eoff[r] = (double)s_E[r] - ex;
+          else
+            s_E[r] = (T)(ex + s_eoff[r]);
+        }
       }
     }
     for (int k = tid; k < R; k += blockDim.x) s_bs[s_bi[k]] = s_beta[k];  // simulation.py:151-152
@@ -496,6 +519,11 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_ais_kernel(
     c.sw = (uint64_t)t;
     const T beta = p.temps[t];
     const double bd = (double)beta;
+    // The reference recomputes the energy after every step
+    // (simulation.py:309); here fields and energy are recomputed in f64
+    // every `refresh` steps and updated by the accepted changes in between,
+    // which bounds the drift of float couplings.
+    if (t > 0 && p.refresh > 0 && t % p.refresh == 0) E = ch.refresh(p.Jt, s_h, n, p.ptype);
     if (p.ordering == 1) {  // Neal's ordering: weight before the transition
       const double refE = (p.ref == 1) ? -((double)(n - ones) * p.lp + (double)ones * p.l1p) : 0.0;
       lw += (-bd * E - (1.0 - bd) * refE) - (-bold * E - (1.0 - bold) * refE);

@@ -32,7 +32,12 @@ enum : uint32_t {
   kStreamPT = 1u,      // one uniform per (read, sweep, replica pair)
   kStreamPerm = 2u,    // Feistel keys per (chain, sweep)
   kStreamShared = 3u,  // Feistel keys per sweep, shared by all chains
+  kStreamBlock = 4u,   // block-random order: block permutation keys per sweep
+  kStreamMask = 5u,    // block-random order: in-block XOR masks per (sweep, block)
 };
+
+// Spins per block of the block-random visit order (and of the MFMA form).
+constexpr int kOrderBlock = 32;
 
 struct u32x4 {
   uint32_t x, y, z, w;
@@ -211,6 +216,35 @@ PYSA_HD u32x4 perm_keys(uint64_t chain, uint64_t sweep, uint32_t purpose, uint32
   c.z = ((uint32_t)chain & 0x00FFFFFFu) | (purpose << 24);
   c.w = (uint32_t)(chain >> 24);
   return philox4x32_10(c, k0, k1);
+}
+
+// Block-random visit order of one sweep, shared by every chain of a run:
+// the nblk = ceil(n / 32) blocks of 32 consecutive positions are visited in
+// the order of a keyed Feistel permutation of [0, nblk), and inside block b
+// the 32 slots k = 0..31 visit position 32 b + (k ^ m_b) for a 5-bit mask
+// m_b (positions >= n are skipped).  The order does not depend on the state,
+// so each single-spin Metropolis update keeps the Boltzmann distribution
+// stationary, as with any random scan.
+//   block keys:  perm_keys(0, sweep, kStreamBlock)
+//   masks:       word b & 3 of Philox((b >> 2, sweep, kStreamMask << 24, 0)), low 5 bits
+PYSA_HD FeistelKey block_order_key(uint32_t nblk, uint64_t sweep, uint32_t k0, uint32_t k1) {
+  return feistel_make(nblk, perm_keys(0ull, sweep, kStreamBlock, k0, k1));
+}
+PYSA_HD uint32_t block_mask(uint64_t sweep, uint32_t b, uint32_t k0, uint32_t k1) {
+  u32x4 c;
+  c.x = b >> 2;
+  c.y = (uint32_t)sweep;
+  c.z = kStreamMask << 24;
+  c.w = 0u;
+  return pick_word(philox4x32_10(c, k0, k1), b & 3u) & 31u;
+}
+// Position visited at step i (0 <= i < 32 * nblk) of the sweep, or -1 for a
+// padding slot of the last block.
+PYSA_HD int block_random_pos(const FeistelKey& f, uint64_t sweep, uint32_t i, int n, uint32_t k0,
+                             uint32_t k1) {
+  const uint32_t b = feistel_perm(f, i >> 5);
+  const int pos = (int)(32u * b + ((i & 31u) ^ block_mask(sweep, b, k0, k1)));
+  return pos < n ? pos : -1;
 }
 
 }  // namespace pysa

@@ -9,8 +9,8 @@ best tracking and log-omega of all reads run in one call into the HIP engine
 Engine options are keyword-only extras, all defaulted so that PySA calls are
 unchanged:
   form        'auto' (blocked MFMA form where eligible, else plain),
-              'plain' or 'blocked'
-  block_size  blocked-form block size B (0 = autotune)
+              'plain', 'blocked' or 'blocked_exact'
+  block_size  blocked-form block size B (0 = the tuned choice)
   seed        Philox key (default: drawn from os.urandom, which leaves the
               global NumPy RNG untouched)
   sweep_offset  Philox sweep-counter offset (resume a stream)
@@ -37,6 +37,7 @@ State = List[float]
 
 _BLOCKED_MIN_N = 32
 _BLOCKED_MAX_N = 2048
+_BLOCKED_RANDOM_MIN_N = 64
 
 
 def _tqdm(it, disable):
@@ -165,7 +166,7 @@ class Solver(object):
 
         init = self._init_strategy(initialize_strategy, init_energies, num_replicas, couplings,
                                    local_fields)
-        if update_strategy not in ('random', 'sequential'):
+        if update_strategy not in ('random', 'sequential', 'block_random'):
             raise ValueError(f"update_strategy='{update_strategy}' not recognized.")
         if int(num_sweeps) < 1:
             raise ValueError("num_sweeps must be >= 1.")
@@ -205,8 +206,8 @@ class Solver(object):
                 if not np.array_equal(np.sort(rep_perm, axis=-1),
                                       np.broadcast_to(np.arange(N, dtype=np.int32), rep_perm.shape)):
                     raise ValueError("replay['perm'] rows must be permutations of range(n).")
-            elif update_strategy == 'random':
-                raise ValueError("replay of update_strategy='random' needs replay['perm'].")
+            elif update_strategy != 'sequential':
+                raise ValueError(f"replay of update_strategy='{update_strategy}' needs replay['perm'].")
             if use_pt and R > 1:
                 if replay.get('pt_logu') is None:
                     raise ValueError("replay with use_pt needs replay['pt_logu'].")
@@ -362,24 +363,33 @@ class Solver(object):
         return _init
 
     def _resolve_form(self, form, update_strategy, ft, R, replay):
-        """Engine form: 'plain' runs every request; 'blocked' / 'blocked_exact'
-        (the MFMA form) need float32, sequential order, Philox randomness,
-        at most 64 replicas and n <= 2048; 'auto' takes the blocked form when
-        it applies."""
+        """Engine form and visit order.
+
+        'plain' runs every request: 'sequential' in position order, 'random'
+        with a fresh permutation per replica and sweep (pysa/simulation.py:53),
+        'block_random' in the block-random order.  The MFMA form
+        ('blocked' / 'blocked_exact') needs float32, Philox randomness, at
+        most 64 replicas and n <= 2048; it runs 'sequential' in position
+        order and 'random' / 'block_random' in the block-random order (a
+        random order of the 32-spin blocks per sweep with a random XOR
+        shuffle inside each block, shared by all replicas; n > 64).  'auto'
+        takes the MFMA form when it applies."""
         if form not in ('auto', 'plain', 'blocked', 'blocked_exact'):
             raise ValueError(f"form='{form}' not recognized.")
+        blocked_order = 'sequential' if update_strategy == 'sequential' else 'block_random'
         ok = (ft == np.float32 and _BLOCKED_MIN_N <= self.n_vars <= _BLOCKED_MAX_N and
-              R <= 64 and replay is None and update_strategy == 'sequential' and
-              _lib.blocked_available())
+              R <= 64 and replay is None and _lib.blocked_available() and
+              (blocked_order == 'sequential' or self.n_vars > _BLOCKED_RANDOM_MIN_N))
         if form == 'auto':
             form = 'blocked' if ok else 'plain'
         if form in ('blocked', 'blocked_exact'):
             if not ok:
                 raise ValueError(
-                    f"form='{form}' needs float_type='float32', update_strategy='sequential', "
-                    f"{_BLOCKED_MIN_N} <= n <= {_BLOCKED_MAX_N}, num_replicas <= 64 and no replay.")
+                    f"form='{form}' needs float_type='float32', {_BLOCKED_MIN_N} <= n <= "
+                    f"{_BLOCKED_MAX_N} (n > {_BLOCKED_RANDOM_MIN_N} for a random order), "
+                    "num_replicas <= 64 and no replay.")
             code = _lib.FORM_BLOCKED if form == 'blocked' else _lib.FORM_BLOCKED_EXACT
-            return code, _lib.ORDER['sequential']
+            return code, _lib.ORDER[blocked_order]
         return _lib.FORM_PLAIN, _lib.ORDER[update_strategy]
 
     def annealed_importance(self,
@@ -461,7 +471,7 @@ class Solver(object):
                 prob.handle, _lib.ptr(np.ascontiguousarray(temps)), int(num_temps),
                 int(num_samps), ref_code, p, order, 0 if ordering == 'reference' else 1,
                 int(seed) & (2**64 - 1), _lib.ptr(rep_lr), _lib.ptr(rep_perm), _lib.ptr(spins),
-                _lib.ptr(log_omegas), _lib.ctypes.byref(kms)))
+                _lib.ptr(log_omegas), None, _lib.ctypes.byref(kms)))
         finally:
             prob.close()
         if engine_stats is not None:

@@ -99,14 +99,16 @@ def assert_same(a, b, keys=("states", "energies", "beta_idx", "best_state", "bes
 
 
 def engine_ais(J, h, ptype, temps, states, *, ref=0, p=0.5, order=0, ordering=0, seed=0,
-               replay=None):
-    """AIS through the C ABI; returns (states, log_w) like oracle.ais."""
+               replay=None, energies=False):
+    """AIS through the C ABI; returns (states, log_w) like oracle.ais, and the
+    tracked final energies as a third item when `energies`."""
     ft = np.dtype(J.dtype)
     prob = _lib.Problem(J, h, "ising" if ptype == 0 else "qubo", ft)
     st = np.ascontiguousarray(states, dtype=np.int8).copy()
     chains = st.shape[0]
     temps = np.ascontiguousarray(temps, dtype=ft)
     lw = np.empty(chains, np.float64)
+    en = np.empty(chains, np.float64)
     lr = pm = None
     if replay is not None:
         lr = np.ascontiguousarray(replay["log_r"], dtype=np.float64)
@@ -117,7 +119,7 @@ def engine_ais(J, h, ptype, temps, states, *, ref=0, p=0.5, order=0, ordering=0,
         _lib.check(_lib.load().pysa_amd_ais(
             prob.handle, _lib.ptr(temps), len(temps), chains, ref, p, order, ordering,
             seed & (2**64 - 1), _lib.ptr(lr), _lib.ptr(pm), _lib.ptr(st), _lib.ptr(lw),
-            _lib.ctypes.byref(kms)))
+            _lib.ptr(en), _lib.ctypes.byref(kms)))
     finally:
         prob.close()
-    return st, lw
+    return (st, lw, en) if energies else (st, lw)

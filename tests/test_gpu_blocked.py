@@ -86,3 +86,82 @@ def test_blocked_bf16_pieces_integer_bit_exact(ptype):
                             form=_lib.FORM_BLOCKED)
     assert ref["accepted"].sum() > 0
     assert_same(got, ref)
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("n,R,reads", [(65, 16, 5), (100, 32, 3), (257, 8, 9), (1024, 32, 2)])
+@pytest.mark.parametrize("best_sem", [0, 1])
+def test_block_random_order_bit_exact(ptype, n, R, reads, best_sem):
+    """The block-random order (update_strategy='random' of the MFMA form):
+    with integer couplings the plain and the blocked kernels both equal the
+    oracle's chain bit for bit."""
+    rng = np.random.default_rng(3000 + n + R + ptype + 5 * best_sem)
+    J, h = int_problem(n, rng, -9, 9, 3)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    S = 25 if n < 1000 else 6
+    betas = (1 / np.geomspace(0.8, 4.0, R)).astype(np.float32)
+    st = random_spins((reads, R, n), rng, ptype)
+    en = energies_of(J, h, st, ptype, np.float32)
+    kw = dict(order=3, seed=55 + n, best_sem=best_sem)
+    ref = oracle.metropolis(J, h, ptype, betas, st, en, S, nthreads=8, **kw)
+    assert ref["accepted"].sum() > 0
+    for form in (_lib.FORM_PLAIN, _lib.FORM_BLOCKED):
+        got = engine_metropolis(J, h, ptype, betas, st, en, S, form=form, **kw)
+        assert_same(got, ref)
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("order", [0, 3])
+def test_headline_kernel_bit_exact(ptype, order):
+    """The benchmarked instantiation (n = 1024: two bf16 pieces, eight spin
+    words per matrix wave, three K ranges) on integer couplings in
+    [-300, 300], which two bf16 pieces hold exactly: bit for bit against the
+    oracle."""
+    rng = np.random.default_rng(77 + ptype + 3 * order)
+    n, R, reads, S = 1024, 32, 3, 8
+    J, h = int_problem(n, rng, -300, 300, 60)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    betas = (1 / np.geomspace(40.0, 900.0, R)).astype(np.float32)
+    st = random_spins((reads, R, n), rng, ptype)
+    en = energies_of(J, h, st, ptype, np.float32)
+    kw = dict(order=order, seed=9001 + order)
+    ref = oracle.metropolis(J, h, ptype, betas, st, en, S, nthreads=8, **kw)
+    acc = ref["accepted"].sum() / (reads * R * n * S)
+    assert 0.02 < acc < 0.98, acc
+    got = engine_metropolis(J, h, ptype, betas, st, en, S, form=_lib.FORM_BLOCKED, **kw)
+    assert_same(got, ref)
+
+
+def _qubo_float(kind, n, rng):
+    if kind == "gauss":
+        Q = np.triu(rng.normal(size=(n, n)), 1)
+        J = Q + Q.T
+        h = rng.normal(size=n)
+    else:  # nonnegative couplings, strongly negative fields: E far from 0
+        Q = np.triu(rng.random((n, n)), 1)
+        J = Q + Q.T
+        h = -rng.uniform(0.2 * n, 0.3 * n, n)
+    return J.astype(np.float32), h.astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["gauss", "nonneg"])
+@pytest.mark.parametrize("order", [0, 3])
+@pytest.mark.parametrize("form", [_lib.FORM_BLOCKED, _lib.FORM_BLOCKED_EXACT])
+def test_blocked_float_qubo(kind, order, form):
+    """QUBO with real couplings on the MFMA form (+-1 operands and row sums of
+    the couplings the products use): tracked and best energies within 1e-5
+    of the f64 energies of the returned states."""
+    rng = np.random.default_rng(11 + order + (kind == "nonneg"))
+    n, R, reads, S = 384, 32, 4, 400
+    J, h = _qubo_float(kind, n, rng)
+    betas = (1 / np.geomspace(0.05, 5.0, R)).astype(np.float32)
+    if kind == "nonneg":
+        betas = (1 / np.geomspace(0.5, 50.0, R)).astype(np.float32)
+    st = random_spins((reads, R, n), rng, 1)
+    en = energies_of(J, h, st, 1, np.float32)
+    got = engine_metropolis(J, h, 1, betas, st, en, S, order=order, seed=4, form=form)
+    e64 = oracle.energy_f64(J, h, got["states"].reshape(-1, n)).reshape(reads, R)
+    rel = np.abs(got["energies"] - e64) / np.maximum(1.0, np.abs(e64))
+    assert rel.max() < 1e-5, rel.max()
+    b64 = oracle.energy_f64(J, h, got["best_state"])
+    assert np.all(np.abs(got["best_energy"] - b64) <= 1e-5 * np.maximum(1.0, np.abs(b64)))

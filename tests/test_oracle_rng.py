@@ -88,6 +88,15 @@ void shim_log(const double* x, double* y, long n) { for (long i = 0; i < n; ++i)
 void shim_stream(uint64_t seed, uint64_t chain, uint64_t sweep, int n, int order, double* lr, int* pm) {
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   for (int p = 0; p < n; ++p) lr[p] = log_u32(pick_word(sweep_block(chain, sweep, p, k0, k1), p & 3));
+  if (order == 3) {
+    FeistelKey f = block_order_key((n + 31) / 32, sweep, k0, k1);
+    int o = 0;
+    for (int i = 0; i < 32 * ((n + 31) / 32); ++i) {
+      const int p = block_random_pos(f, sweep, i, n, k0, k1);
+      if (p >= 0) pm[o++] = p;
+    }
+    return;
+  }
   FeistelKey f = feistel_make(n, perm_keys(order == 2 ? 0 : chain, sweep,
                                            order == 2 ? kStreamShared : kStreamPerm, k0, k1));
   for (int i = 0; i < n; ++i) pm[i] = order == 0 ? i : (int)feistel_perm(f, i);
@@ -127,7 +136,7 @@ def test_product_header_matches_oracle(product_rng):
     y = np.empty_like(x)
     product_rng.shim_log(_p(x), _p(y), x.size)
     assert np.array_equal(y, oracle.det_log(x))
-    for order in (0, 1, 2):
+    for order in (0, 1, 2, 3):
         for n, chain, sweep in ((5, 0, 0), (130, 77, 12), (1024, 2**30 + 5, 2**33 + 1)):
             lr = np.empty(n)
             pm = np.empty(n, np.int32)
@@ -135,3 +144,33 @@ def test_product_header_matches_oracle(product_rng):
             olr, opm = oracle.sweep_stream(99, chain, sweep, n, order)
             assert np.array_equal(lr, olr)
             assert np.array_equal(pm, opm)
+
+
+@pytest.mark.parametrize("n", [1, 31, 32, 33, 65, 100, 1024, 2048])
+def test_block_random_order_structure(n):
+    """The block-random order (order 3) from its definition: every sweep
+    visits each position once; blocks of 32 consecutive positions are visited
+    whole, one after another, in a keyed order; inside block b slot k visits
+    32 b + (k ^ m_b); the order is the same for every chain and changes with
+    the sweep."""
+    nblk = (n + 31) // 32
+    seen_orders = set()
+    for sweep in range(6):
+        _, pm = oracle.sweep_stream(7, 0, sweep, n, 3)
+        assert sorted(pm.tolist()) == list(range(n))
+        _, pm2 = oracle.sweep_stream(7, 12345, sweep, n, 3)
+        assert np.array_equal(pm, pm2)  # shared by every chain
+        blocks = pm // 32
+        # blocks appear as contiguous runs, each block once
+        runs = [blocks[0]] + [b for a, b in zip(blocks[:-1], blocks[1:]) if a != b]
+        assert sorted(runs) == list(range(nblk))
+        for b in range(nblk):
+            got = pm[blocks == b] - 32 * b
+            nb = min(32, n - 32 * b)
+            # slot k visits k ^ m for some mask m; slots past n are skipped
+            masks = [m for m in range(32)
+                     if [k ^ m for k in range(32) if (k ^ m) < nb] == got.tolist()]
+            assert masks, (b, got)
+        seen_orders.add(tuple(pm.tolist()))
+    if n > 1:
+        assert len(seen_orders) > 1

@@ -288,6 +288,65 @@ def test_detailed_balance_per_slot():
         assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (k, chi2)
 
 
+@pytest.mark.parametrize("order", [1, 3])
+def test_detailed_balance_ring_many_blocks(order):
+    """Stationarity over several 32-spin blocks (n = 70, three blocks, the
+    last one padded): a ferromagnetic ring E = -sum_i s_i s_(i+1) has
+    exactly 2 C(n, k) states with k broken bonds (k even), so the number of
+    broken bonds in PT slot k follows P(k) ~ C(n, k) exp(-2 beta k).
+    (Sequential order is left out: its zero-cost domain-wall moves carry
+    every wall along the sweep direction, so walls rarely meet and the
+    chain mixes too slowly for a 60-sweep sample.)"""
+    rng = np.random.default_rng(70 + order)
+    n, R, reads, S = 70, 4, 1500, 60
+    J = np.zeros((n, n))
+    for i in range(n):
+        J[i, (i + 1) % n] = J[(i + 1) % n, i] = -1.0
+    h = np.zeros(n)
+    betas = np.array([0.9, 0.6, 0.4, 0.25], dtype=np.float32)
+    st = 2 * rng.integers(0, 2, (reads, R, n)) - 1
+    out = run(J, h, 0, betas, st, S, order=order, seed=17, nthreads=8)
+    ks = np.arange(0, n + 1, 2)
+    for k in range(R):
+        logw = np.array([math.lgamma(n + 1) - math.lgamma(b + 1) - math.lgamma(n - b + 1)
+                         for b in ks]) - 2.0 * float(betas[k]) * ks
+        p = np.exp(logw - logw.max())
+        p /= p.sum()
+        samp = out["energies"][np.arange(reads), out["beta_idx"][:, k]]
+        broken = np.rint((samp + n) / 2).astype(int)
+        obs = np.array([(broken == b).sum() for b in ks])
+        assert obs.sum() == reads
+        keep = p * reads >= 5
+        exp_ = p[keep] * reads
+        o = obs[keep]
+        chi2 = ((o - exp_ * o.sum() / exp_.sum()) ** 2 / (exp_ * o.sum() / exp_.sum())).sum()
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (k, chi2)
+
+
+def test_detailed_balance_per_slot_block_random():
+    """The per-slot chi-square of test_detailed_balance_per_slot for the
+    block-random order inside one padded block."""
+    rng = np.random.default_rng(44)
+    n, R, reads, S = 6, 4, 3000, 40
+    A = np.triu(rng.integers(-2, 3, (n, n)), 1)
+    J, h = A + A.T, rng.integers(-1, 2, n)
+    betas = np.array([1.0, 0.7, 0.45, 0.25], dtype=np.float32)
+    st = 2 * rng.integers(0, 2, (reads, R, n)) - 1
+    out = run(J, h, 0, betas, st, S, order=3, seed=12, nthreads=8)
+    allE = energies_np(J, h, all_states(n, 0))
+    levels = np.unique(allE)
+    for k in range(R):
+        w = np.exp(-float(betas[k]) * (allE - allE.min()))
+        p = np.array([w[allE == L].sum() for L in levels]) / w.sum()
+        samp = out["energies"][np.arange(reads), out["beta_idx"][:, k]]
+        obs = np.array([(samp == L).sum() for L in levels])
+        keep = p * reads >= 5
+        exp_ = p[keep] * reads
+        o = obs[keep]
+        chi2 = ((o - exp_ * o.sum() / exp_.sum()) ** 2 / (exp_ * o.sum() / exp_.sum())).sum()
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-4, (k, chi2)
+
+
 def rbm_problem(nv, nh, seed, ws=0.3, bs=0.2):
     rng = np.random.default_rng(seed)
     W = rng.normal(0, ws, (nv, nh))
