@@ -124,6 +124,30 @@ int pysa_amd_metropolis_device(int64_t handle, int32_t reads, int32_t R, int32_t
                                double* kernel_ms, void* stream);
 
 /*
+ * pysa_amd_metropolis_device for the MFMA form (PYSA_AMD_FORM_BLOCKED or
+ * _BLOCKED_EXACT) that allocates nothing: no device memory, no event, no
+ * synchronisation.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).
+ *   - The problem's couplings must already be encoded for the form: run the
+ *     same form once through pysa_amd_metropolis(_device) on the handle.
+ *   - The per-replica best states live in a static device array of 32 MiB
+ *     per GPU (held until the process ends): reads * R * (n + 8) bytes must
+ *     fit, else PYSA_AMD_EINVAL and nothing is launched.
+ *   - The work is enqueued on `stream` (NULL = the default stream) and the
+ *     call returns at once; the caller synchronises.  Calls that overlap on
+ *     one GPU share that array and race.
+ * Arguments as in pysa_amd_metropolis_device (no replay, no timing).
+ */
+int pysa_amd_metropolis_device_noalloc(int64_t handle, int32_t reads, int32_t R, int32_t S,
+                                       int32_t order, int32_t use_pt, int32_t log_omega,
+                                       int32_t best_semantics, const void* betas,
+                                       const int32_t* beta_sort, uint64_t seed,
+                                       uint64_t sweep_offset, int32_t form, int8_t* states,
+                                       void* energies, int64_t* beta_idx, int8_t* best_state,
+                                       void* best_energy, int32_t* best_sweeps,
+                                       double* log_omegas, uint64_t* accepted, void* stream);
+
+/*
  * Annealed importance sampling over `chains` independent chains
  * (pysa/simulation.py:266-317 with update_spin_ref, pysa/ising.py:66-92).
  * The caller passes J and h already scaled by the target beta

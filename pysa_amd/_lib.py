@@ -37,6 +37,9 @@ _SIGNATURES = {
                                    _c_i32, _vp, _vp, _c_u64, _c_u64, _c_i32, _c_i32, _vp, _vp,
                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                    ctypes.POINTER(_c_dbl), _vp],
+    "pysa_amd_metropolis_device_noalloc": [_c_i64, _c_i32, _c_i32, _c_i32, _c_i32, _c_i32,
+                                           _c_i32, _c_i32, _vp, _vp, _c_u64, _c_u64, _c_i32,
+                                           _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pysa_amd_ais": [_c_i64, _vp, _c_i32, _c_i32, _c_i32, _c_dbl, _c_i32, _c_i32, _c_u64,
                      _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],

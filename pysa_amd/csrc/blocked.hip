@@ -45,10 +45,18 @@
 #include <utility>
 #include <vector>
 
+#include <mutex>
+
 #include "../../include/pysa_amd.h"
 #include "accept.hpp"
 #include "engine.hpp"
 #include "rng.hpp"
+
+// Best-state scratch of the allocation-free entry point
+// (pysa_amd_metropolis_device_noalloc): one copy per device, held until the
+// process ends.  Host code never uses this name as an address: it passes the
+// device address that noalloc_scratch() looks up.
+static __device__ __align__(256) uint8_t scratch[pysa::kNoAllocScratchBytes];
 
 namespace pysa {
 namespace {
@@ -1043,6 +1051,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   // rsb (zero otherwise); fsc is a power of two, so the fma rounds like the add
   const bool qsum = !I8 && p.ptype == 1;
   const float fsc = qsum ? 0.5f : 1.f;
+
   uint32_t t = 0;                               // running block counter
   // The two roles run separate copies of the block loop (so that each keeps
   // only its own state in registers); both pass the same barriers: one after
@@ -1413,24 +1422,23 @@ int blocked_exact_pieces(const float* J, size_t count) {
   return i8 ? 0 : np;
 }
 
-int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* dh, int n, int ptype, int dtype,
-                int reads, int R, int S, int order, int use_pt, int logw, int best_sem, int pieces,
-                uint64_t seed, uint64_t sweep_offset, const void* betas, const int32_t* bsort,
-                const double* rep_lr, const double* rep_pt, int8_t* states, void* energies,
-                int64_t* beta_idx, int8_t* best_state, void* best_energy, int32_t* best_sweeps,
-                double* log_omegas, uint64_t* accepted, hipStream_t st, double* ms,
-                std::string* why) {
-  if (!blocked_eligible(n, dtype, R, reads, order, rep_lr != nullptr || rep_pt != nullptr)) {
-    *why = "blocked form needs float32 couplings, 32 <= n <= 2048, R <= 64, sequential or "
-           "block-random order (the latter for n > 64) and Philox randomness (no replay)";
-    return PYSA_AMD_ENOTSUP;
-  }
-  const int nblk = (n + kB - 1) / kB;
+namespace {
+
+// Decisions of one run that depend only on its shape and the encoding.
+struct BlockedPlan {
+  int nblk, groups, kw, kb, nw, kbp, nwp, kind, np;
+  bool pipe, i8, inexact;
+};
+
+// Coupling encoding and kernel shape; b.exact_kind must be known.
+BlockedPlan plan_blocked(const BlockedProblem& b, int n, int reads, int R, int order, int pieces) {
+  BlockedPlan pl;
+  pl.nblk = (n + kB - 1) / kB;
   // Waves per replica group: enough groups x waves to give every SIMD of the
   // device work (1024 on MI355X), at most 4, at least one block per wave.
-  const int groups = (reads + (64 / R) - 1) / (64 / R);
-  int kw = 1;
-  while (kw < 4 && 2 * kw <= nblk && groups * kw < 1024) kw *= 2;
+  pl.groups = (reads + (64 / R) - 1) / (64 / R);
+  pl.kw = 1;
+  while (pl.kw < 4 && 2 * pl.kw <= pl.nblk && pl.groups * pl.kw < 1024) pl.kw *= 2;
   // The pipelined kernel (proposal wave + six matrix waves) takes every
   // problem of at least three blocks; PYSA_AMD_BLOCKED_SPLITK=1 selects the
   // split-K kernel instead (kept for small problems and comparisons).
@@ -1438,22 +1446,14 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     const char* e = getenv("PYSA_AMD_BLOCKED_SPLITK");
     return e && e[0] == '1';
   }();
-  const bool pipe = nblk >= kPipeMW && (!force_splitk || order == kBlockRandom);
-  const int kbp = (nblk + kPipeMW - 1) / kPipeMW;
-  int nwp = 4;
-  while (2 * nwp < kbp) nwp *= 2;
-  const int kb = (nblk + kw - 1) / kw;
-  int nw = 4;
-  while (2 * nw < kb) nw *= 2;  // words of 2 blocks each: kb <= 2 * nw
+  pl.pipe = pl.nblk >= kPipeMW && (!force_splitk || order == kBlockRandom);
+  pl.kbp = (pl.nblk + kPipeMW - 1) / kPipeMW;
+  pl.nwp = 4;
+  while (2 * pl.nwp < pl.kbp) pl.nwp *= 2;
+  pl.kb = (pl.nblk + pl.kw - 1) / pl.kw;
+  pl.nw = 4;
+  while (2 * pl.nw < pl.kb) pl.nw *= 2;  // words of 2 blocks each: kb <= 2 * nw
   // Coupling encoding: exact where possible; `pieces` forces 2 or 3 bf16 pieces.
-  if (b.exact_kind < 0) {
-    std::vector<float> hJ((size_t)n * n);
-    if (hipMemcpy(hJ.data(), dJ, hJ.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
-      *why = "reading couplings back";
-      return PYSA_AMD_EHIP;
-    }
-    b.exact_kind = blocked_exact_pieces(hJ.data(), hJ.size());
-  }
   int kind = b.exact_kind;  // 0 int8, else bf16 pieces
   if (kind != 0) {
     if (pieces == 2 || pieces == 3)
@@ -1462,36 +1462,56 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
       kind = (kind > 2) ? 2 : kind;  // default: hi+lo split for real couplings
     if (pieces == 3 && b.exact_kind >= 3) kind = 3;
   }
-  const bool i8 = kind == 0;
-  const int np = i8 ? 1 : kind;
+  pl.kind = kind;
+  pl.i8 = kind == 0;
+  pl.np = pl.i8 ? 1 : kind;
+  // The pieces do not hold every coupling exactly: the in-block updates and
+  // the QUBO row sums then use the piece sums too (the couplings the products
+  // multiply with), so all fields derive from one coupling matrix.
+  pl.inexact = !pl.i8 && kind < b.exact_kind;
+  return pl;
+}
+
+// Whether the problem's device copies match the plan (packed couplings,
+// piece sums, row sums).
+bool encoding_ready(const BlockedProblem& b, int ptype, const BlockedPlan& pl) {
+  if (!b.packed || b.packed_kind != pl.kind) return false;
+  if (pl.inexact && (!b.jtl || b.jtl_kind != pl.kind)) return false;
+  if (!pl.i8 && ptype == 1 && (!b.rowsum || b.rowsum_kind != pl.kind)) return false;
+  return true;
+}
+
+// Build the device copies the plan needs (allocates on first use).
+int prepare_encoding(BlockedProblem& b, const void* dJ, const void* dJt, int n, int ptype,
+                     const BlockedPlan& pl, hipStream_t st, std::string* why) {
+  const int nblk = pl.nblk, kind = pl.kind, np = pl.np;
   if (b.packed_kind != kind || !b.packed) {
     if (b.packed) (void)hipFree(b.packed);
     b.packed = nullptr;
-    const size_t bytes = i8 ? (size_t)nblk * nblk * 64 * 16 : (size_t)nblk * 2 * nblk * np * 64 * 16;
+    b.packed_kind = -1;
+    const size_t bytes =
+        pl.i8 ? (size_t)nblk * nblk * 64 * 16 : (size_t)nblk * 2 * nblk * np * 64 * 16;
     if (hipMalloc(&b.packed, bytes) != hipSuccess) {
+      b.packed = nullptr;
       *why = "allocating packed couplings";
       return PYSA_AMD_EHIP;
     }
     b.packed_bytes = bytes;
     const long long threads = (long long)(bytes / 16);
     const unsigned nb = (unsigned)((threads + 255) / 256);
-    if (i8)
+    if (pl.i8)
       hipLaunchKernelGGL(pack_i8_kernel, dim3(nb), dim3(256), 0, st, (const float*)dJ, n, nblk,
                          (int8_t*)b.packed);
     else
-      hipLaunchKernelGGL(pack_bf16_kernel, dim3(nb), dim3(256), 0, st, (const float*)dJ, n, nblk, np,
-                         (uint16_t*)b.packed);
+      hipLaunchKernelGGL(pack_bf16_kernel, dim3(nb), dim3(256), 0, st, (const float*)dJ, n, nblk,
+                         np, (uint16_t*)b.packed);
     if (hipGetLastError() != hipSuccess) {
       *why = "packing couplings";
       return PYSA_AMD_EHIP;
     }
     b.packed_kind = kind;
   }
-  // The pieces do not hold every coupling exactly: the in-block updates and
-  // the QUBO row sums then use the piece sums too (the couplings the products
-  // multiply with), so all fields derive from one coupling matrix.
-  const bool inexact = !i8 && kind < b.exact_kind;
-  if (inexact && b.jtl_kind != kind) {
+  if (pl.inexact && b.jtl_kind != kind) {
     if (!b.jtl && hipMalloc(&b.jtl, sizeof(float) * (size_t)n * n) != hipSuccess) {
       b.jtl = nullptr;
       *why = "allocating piece-sum couplings";
@@ -1506,119 +1526,260 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     }
     b.jtl_kind = kind;
   }
-  if (!i8 && ptype == 1 && b.rowsum_kind != kind) {
+  if (!pl.i8 && ptype == 1 && b.rowsum_kind != kind) {
     if (!b.rowsum && hipMalloc(&b.rowsum, sizeof(float) * n) != hipSuccess) {
       b.rowsum = nullptr;
       *why = "allocating row sums";
       return PYSA_AMD_EHIP;
     }
-    hipLaunchKernelGGL(rowsum_kernel, dim3((n + 255) / 256), dim3(256), 0, st, (const float*)dJ, n,
-                       inexact ? np : 0, b.rowsum);
+    hipLaunchKernelGGL(rowsum_kernel, dim3((n + 255) / 256), dim3(256), 0, st, (const float*)dJ,
+                       n, pl.inexact ? np : 0, b.rowsum);
     if (hipGetLastError() != hipSuccess) {
       *why = "row sums";
       return PYSA_AMD_EHIP;
     }
     b.rowsum_kind = kind;
   }
+  return 0;
+}
+
+// Best-state scratch of a run: reads*R*n spins, then reads*R energies and
+// reads*R sweep indices.
+size_t scratch_need(int n, int reads, int R) {
   const size_t cnt = (size_t)reads * R;
-  const size_t need = cnt * n + cnt * 4 + cnt * 4;
-  if (b.scratch_bytes < need) {
-    if (b.scratch) (void)hipFree(b.scratch);
-    b.scratch = nullptr;
-    if (hipMalloc(&b.scratch, need) != hipSuccess) {
-      *why = "allocating scratch";
-      return PYSA_AMD_EHIP;
-    }
-    b.scratch_bytes = need;
-  }
+  return cnt * n + cnt * 4 + cnt * 4;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0ull;
+}
+
+// Enqueue one run on stream st (no allocation, no synchronisation).
+hipError_t launch_blocked(const BlockedProblem& b, const BlockedPlan& pl, const void* dJt,
+                          const void* dh, int n, int ptype, int reads, int R, int S, int order,
+                          int use_pt, int logw, int best_sem, uint64_t seed, uint64_t sweep_offset,
+                          const void* betas, const int32_t* bsort, int8_t* states, void* energies,
+                          int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                          int32_t* best_sweeps, double* log_omegas, uint64_t* accepted,
+                          void* scratch_mem, hipStream_t st) {
+  const size_t cnt = (size_t)reads * R;
   BParams p;
   p.n = n;
-  p.nblk = nblk;
+  p.nblk = pl.nblk;
   p.ptype = ptype;
   p.R = R;
   p.reads = reads;
   p.S = S;
   p.rpw = 64 / R;
-  p.np = np;
+  p.np = pl.np;
   p.use_pt = use_pt;
   p.logw = logw;
   p.best_sem = best_sem;
-  p.kb = pipe ? kbp : kb;
+  p.kb = pl.pipe ? pl.kbp : pl.kb;
   p.k0 = (uint32_t)seed;
   p.k1 = (uint32_t)(seed >> 32);
   p.sweep_offset = sweep_offset;
   p.frag = b.packed;
-  p.Jt = inexact ? b.jtl : (const float*)dJt;
+  p.Jt = pl.inexact ? b.jtl : (const float*)dJt;
   p.h = (const float*)dh;
   p.order = order;
-  p.rowsum = (!i8 && ptype == 1) ? b.rowsum : nullptr;
+  p.rowsum = (!pl.i8 && ptype == 1) ? b.rowsum : nullptr;
   p.betas = (const float*)betas;
   p.bsort = bsort;
   p.spins = states;
   p.energies = (float*)energies;
   p.beta_idx = beta_idx;
-  p.best_rep = (int8_t*)b.scratch;
-  p.best_rep_e = (float*)((int8_t*)b.scratch + cnt * n);
-  p.best_rep_sw = (int*)((int8_t*)b.scratch + cnt * n + cnt * 4);
+  p.best_rep = (int8_t*)scratch_mem;
+  p.best_rep_e = (float*)((int8_t*)scratch_mem + cnt * n);
+  p.best_rep_sw = (int*)((int8_t*)scratch_mem + cnt * n + cnt * 4);
   p.best_state = best_state;
   p.best_energy = (float*)best_energy;
   p.best_sweeps = best_sweeps;
   p.log_omegas = logw ? log_omegas : nullptr;
   p.accepted = (unsigned long long*)accepted;
-  if (best_sem == 0 &&
-      hipMemcpyAsync(p.best_rep, states, cnt * n, hipMemcpyDeviceToDevice, st) != hipSuccess) {
-    *why = "initialising best states";
+  hipError_t e = hipSuccess;
+  if (best_sem == 0) {
+    e = hipMemcpyAsync(p.best_rep, states, cnt * n, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  if (accepted) {  // a fill kernel, not a memset, so that the run can be captured in a graph
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((reads + 255) / 256), dim3(256), 0, st,
+                       (unsigned long long*)accepted, reads);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int np = pl.np;
+  if (pl.pipe) {
+    if (pl.i8)
+      e = launch_pipe<1, true>(p, pl.nwp, pl.groups, st);
+    else if (np == 1)
+      e = launch_pipe<1, false>(p, pl.nwp, pl.groups, st);
+    else if (np == 2)
+      e = launch_pipe<2, false>(p, pl.nwp, pl.groups, st);
+    else
+      e = launch_pipe<3, false>(p, pl.nwp, pl.groups, st);
+  } else if (pl.i8)
+    e = launch_np<1, true>(p, pl.nw, pl.kw, pl.groups, st);
+  else if (np == 1)
+    e = launch_np<1, false>(p, pl.nw, pl.kw, pl.groups, st);
+  else if (np == 2)
+    e = launch_np<2, false>(p, pl.nw, pl.kw, pl.groups, st);
+  else
+    e = launch_np<3, false>(p, pl.nw, pl.kw, pl.groups, st);
+  if (e == hipSuccess && best_sem == 0) {
+    hipLaunchKernelGGL(best_argmin_kernel, dim3((reads + 63) / 64), dim3(64), 0, st, p);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    return e;
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+const char* kNotEligible =
+    "blocked form needs float32 couplings, 32 <= n <= 2048, R <= 64, sequential or "
+    "block-random order (the latter for n > 64) and Philox randomness (no replay)";
+
+}  // namespace
+
+// The allocation-free entry point's scratch on each device (a copy of the
+// file-scope array `scratch` per device), looked up once per device with that
+// device current.
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch(void** ptr, size_t* bytes, std::string* why) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    *why = "no current HIP device (or more than 64 devices)";
     return PYSA_AMD_EHIP;
   }
-  if (accepted && hipMemsetAsync(accepted, 0, sizeof(uint64_t) * reads, st) != hipSuccess) {
-    *why = "clearing counters";
-    return PYSA_AMD_EHIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *why = std::string("looking up the allocation-free scratch: ") + hipGetErrorString(e);
+      return PYSA_AMD_EHIP;
+    }
+    g_scratch_ptr[dev] = p;
   }
-  hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+  *ptr = g_scratch_ptr[dev];
+  *bytes = kNoAllocScratchBytes;
+  return 0;
+}
+
+int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* dh, int n,
+                int ptype, int dtype, int reads, int R, int S, int order, int use_pt, int logw,
+                int best_sem, int pieces, uint64_t seed, uint64_t sweep_offset, const void* betas,
+                const int32_t* bsort, const double* rep_lr, const double* rep_pt, int8_t* states,
+                void* energies, int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                int32_t* best_sweeps, double* log_omegas, uint64_t* accepted, hipStream_t st,
+                double* ms, std::string* why) {
+  if (!blocked_eligible(n, dtype, R, reads, order, rep_lr != nullptr || rep_pt != nullptr)) {
+    *why = kNotEligible;
+    return PYSA_AMD_ENOTSUP;
+  }
+  if (b.exact_kind < 0) {
+    std::vector<float> hJ((size_t)n * n);
+    if (hipMemcpy(hJ.data(), dJ, hJ.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      *why = "reading couplings back";
+      return PYSA_AMD_EHIP;
+    }
+    b.exact_kind = blocked_exact_pieces(hJ.data(), hJ.size());
+  }
+  const BlockedPlan pl = plan_blocked(b, n, reads, R, order, pieces);
+  int rc = prepare_encoding(b, dJ, dJt, n, ptype, pl, st, why);
+  if (rc) return rc;
+  const size_t need = scratch_need(n, reads, R);
+  if (b.scratch_bytes < need) {
+    if (b.scratch) (void)hipFree(b.scratch);
+    b.scratch = nullptr;
+    b.scratch_bytes = 0;
+    if (hipMalloc(&b.scratch, need) != hipSuccess) {
+      b.scratch = nullptr;
+      *why = "allocating scratch";
+      return PYSA_AMD_EHIP;
+    }
+    b.scratch_bytes = need;
+  }
+  EventPair ev;
+  if (ev.create() != hipSuccess) {
     *why = "creating events";
     return PYSA_AMD_EHIP;
   }
-  (void)hipEventRecord(e0, st);
-  hipError_t le;
-  if (pipe) {
-    if (i8)
-      le = launch_pipe<1, true>(p, nwp, groups, st);
-    else if (np == 1)
-      le = launch_pipe<1, false>(p, nwp, groups, st);
-    else if (np == 2)
-      le = launch_pipe<2, false>(p, nwp, groups, st);
-    else
-      le = launch_pipe<3, false>(p, nwp, groups, st);
-  } else if (i8)
-    le = launch_np<1, true>(p, nw, kw, groups, st);
-  else if (np == 1)
-    le = launch_np<1, false>(p, nw, kw, groups, st);
-  else if (np == 2)
-    le = launch_np<2, false>(p, nw, kw, groups, st);
-  else
-    le = launch_np<3, false>(p, nw, kw, groups, st);
-  if (le == hipSuccess && best_sem == 0) {
-    hipLaunchKernelGGL(best_argmin_kernel, dim3((reads + 63) / 64), dim3(64), 0, st, p);
-    le = hipGetLastError();
-  }
-  (void)hipEventRecord(e1, st);
+  (void)hipEventRecord(ev.a, st);
+  hipError_t le = launch_blocked(b, pl, dJt, dh, n, ptype, reads, R, S, order, use_pt, logw,
+                                 best_sem, seed, sweep_offset, betas, bsort, states, energies,
+                                 beta_idx, best_state, best_energy, best_sweeps, log_omegas,
+                                 accepted, b.scratch, st);
+  (void)hipEventRecord(ev.b, st);
   if (le != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     *why = std::string("blocked kernel launch: ") + hipGetErrorString(le);
     return PYSA_AMD_EHIP;
   }
-  if (hipEventSynchronize(e1) != hipSuccess) {
+  if (hipEventSynchronize(ev.b) != hipSuccess) {
     *why = "blocked kernel execution";
     return PYSA_AMD_EHIP;
   }
   float t = 0.f;
-  (void)hipEventElapsedTime(&t, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipEventElapsedTime(&t, ev.a, ev.b);
   if (ms) *ms = t;
-  b.last_kind = kind;
+  b.last_kind = pl.kind;
+  return 0;
+}
+
+int run_blocked_noalloc(const BlockedProblem& b, const void* dJt, const void* dh, int n,
+                        int ptype, int dtype, int reads, int R, int S, int order, int use_pt,
+                        int logw, int best_sem, int pieces, uint64_t seed, uint64_t sweep_offset,
+                        const void* betas, const int32_t* bsort, int8_t* states, void* energies,
+                        int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                        int32_t* best_sweeps, double* log_omegas, uint64_t* accepted,
+                        void* scratch_mem, size_t scratch_bytes, hipStream_t st,
+                        std::string* why) {
+  if (!blocked_eligible(n, dtype, R, reads, order, false)) {
+    *why = kNotEligible;
+    return PYSA_AMD_ENOTSUP;
+  }
+  const size_t need = scratch_need(n, reads, R);
+  if (need > scratch_bytes) {
+    *why = "reads * replicas * (n + 8) = " + std::to_string(need) +
+           " bytes exceeds the allocation-free scratch of " + std::to_string(scratch_bytes) +
+           " bytes";
+    return PYSA_AMD_EINVAL;
+  }
+  if (b.exact_kind < 0) {
+    *why = "couplings not yet encoded for the MFMA form: run this form once through an "
+           "allocating entry point first";
+    return PYSA_AMD_ENOTSUP;
+  }
+  const BlockedPlan pl = plan_blocked(b, n, reads, R, order, pieces);
+  if (!encoding_ready(b, ptype, pl)) {
+    *why = "couplings encoded for another form: run this form once through an allocating "
+           "entry point first";
+    return PYSA_AMD_ENOTSUP;
+  }
+  hipError_t le = launch_blocked(b, pl, dJt, dh, n, ptype, reads, R, S, order, use_pt, logw,
+                                 best_sem, seed, sweep_offset, betas, bsort, states, energies,
+                                 beta_idx, best_state, best_energy, best_sweeps, log_omegas,
+                                 accepted, scratch_mem, st);
+  if (le != hipSuccess) {
+    *why = std::string("blocked kernel launch: ") + hipGetErrorString(le);
+    return PYSA_AMD_EHIP;
+  }
   return 0;
 }
 

@@ -10,6 +10,11 @@ namespace pysa {
 // Largest problem the blocked form holds (spin bits of 64 replicas in VGPRs).
 constexpr int kBlockedMaxN = 2048;
 
+// Scratch size of the allocation-free entry point (best states of
+// reads * R chains of n spins plus 8 bytes per chain), and devices it serves.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+
 // Packed couplings and scratch of the blocked form, owned by a problem.
 struct BlockedProblem {
   void* packed = nullptr;  // MFMA A-operand fragments of J
@@ -40,5 +45,22 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
                 void* energies, int64_t* beta_idx, int8_t* best_state, void* best_energy,
                 int32_t* best_sweeps, double* log_omegas, uint64_t* accepted, hipStream_t st,
                 double* ms, std::string* why);
+
+// The same run with no allocation, no event and no synchronisation: the
+// couplings must already be encoded for this form (by an earlier run_blocked
+// on the problem), the best-state scratch is `scratch_mem` (scratch_bytes
+// long), and the work is enqueued on st.
+int run_blocked_noalloc(const BlockedProblem& b, const void* dJt, const void* dh, int n,
+                        int ptype, int dtype, int reads, int R, int S, int order, int use_pt,
+                        int logw, int best_sem, int pieces, uint64_t seed, uint64_t sweep_offset,
+                        const void* betas, const int32_t* bsort, int8_t* states, void* energies,
+                        int64_t* beta_idx, int8_t* best_state, void* best_energy,
+                        int32_t* best_sweeps, double* log_omegas, uint64_t* accepted,
+                        void* scratch_mem, size_t scratch_bytes, hipStream_t st,
+                        std::string* why);
+
+// Device address and size of the allocation-free scratch on the current
+// device (looked up on the first call per device).
+int noalloc_scratch(void** ptr, size_t* bytes, std::string* why);
 
 }  // namespace pysa

@@ -367,6 +367,41 @@ int pysa_amd_metropolis_device(int64_t handle, int32_t reads, int32_t R, int32_t
   return metropolis_device(handle, a, kernel_ms, stream);
 }
 
+int pysa_amd_metropolis_device_noalloc(int64_t handle, int32_t reads, int32_t R, int32_t S,
+                                       int32_t order, int32_t use_pt, int32_t log_omega,
+                                       int32_t best_semantics, const void* betas,
+                                       const int32_t* beta_sort, uint64_t seed,
+                                       uint64_t sweep_offset, int32_t form, int8_t* states,
+                                       void* energies, int64_t* beta_idx, int8_t* best_state,
+                                       void* best_energy, int32_t* best_sweeps,
+                                       double* log_omegas, uint64_t* accepted, void* stream) {
+  g_err.clear();
+  Problem* P = get(handle);
+  RunArgs a = {reads, R, S, order, use_pt, log_omega, best_semantics, form, 0, seed,
+               sweep_offset, betas, beta_sort, nullptr, nullptr, nullptr, states,
+               energies, beta_idx, best_state, best_energy, best_sweeps, log_omegas, accepted};
+  int rc = check_run(P, a);
+  if (rc) return rc;
+  if (form != PYSA_AMD_FORM_BLOCKED && form != PYSA_AMD_FORM_BLOCKED_EXACT)
+    return fail(PYSA_AMD_ENOTSUP, "the allocation-free entry point runs the MFMA form only "
+                                  "(PYSA_AMD_FORM_BLOCKED or PYSA_AMD_FORM_BLOCKED_EXACT)");
+  std::lock_guard<std::mutex> lk(P->mu);
+  HIP_TRY(hipSetDevice(P->device));
+  void* sp = nullptr;
+  size_t sb = 0;
+  std::string why;
+  rc = pysa::noalloc_scratch(&sp, &sb, &why);
+  if (rc) return fail(rc, why);
+  rc = pysa::run_blocked_noalloc(P->blocked, P->dJt, P->dh, P->n, P->ptype, P->dtype, reads, R, S,
+                                 order, use_pt, log_omega, best_semantics,
+                                 form == PYSA_AMD_FORM_BLOCKED_EXACT ? 3 : 0, seed, sweep_offset,
+                                 betas, beta_sort, states, energies, beta_idx, best_state,
+                                 best_energy, best_sweeps, log_omegas, accepted, sp, sb,
+                                 (hipStream_t)stream, &why);
+  if (rc) return fail(rc, why);
+  return 0;
+}
+
 int pysa_amd_metropolis(int64_t handle, int32_t reads, int32_t R, int32_t S, int32_t order,
                         int32_t use_pt, int32_t log_omega, int32_t best_semantics,
                         const void* betas, const int32_t* beta_sort, uint64_t seed,
