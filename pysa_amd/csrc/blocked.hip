@@ -266,6 +266,7 @@ __device__ __forceinline__ void set_half(Bits<NW>& s, int c, uint32_t v) {
 struct BParams {
   int n, nblk, ptype, R, reads, S, rpw, np, use_pt, logw, best_sem;
   int order;  // kSequential or kBlockRandom (pipelined kernel only)
+  int refresh;  // pipelined kernel: sweeps between f64 energy recomputations (0 = never)
   int kb;  // row blocks of the K range each wave of a group owns
   uint32_t k0, k1;
   uint64_t sweep_offset;
@@ -844,7 +845,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     // jn passes through an empty asm so that the per-stage offsets are
     // recomputed (a few scalar ops) rather than hoisted out of the block
     // loop into registers that would spill
-    int jnl = jn;
+    int jnl = __builtin_amdgcn_readfirstlane(jn);
     asm volatile("" : "+s"(jnl));
     const int jc = jnl > 0 ? min(j, jnl - 1) : nblk - 1 - c0;
     // wave-uniform base of the stage, then a per-lane 16-byte offset
@@ -949,7 +950,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       if (jb0 >= jn) break;
       // stages left from this chunk on, recomputed per chunk (a scalar
       // compare a slot instead of per-slot lane masks kept across the loop)
-      int jrem = jn - jb0;
+      int jrem = __builtin_amdgcn_readfirstlane(jn - jb0);
       asm volatile("" : "+s"(jrem));
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
@@ -1007,11 +1008,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
 
   // Visit schedule of sweep counter swn into parity par (wave 4, one lane
   // per block): the block visited at step b and the in-block mask of block b.
-  auto fill_sched = [&](int par, uint64_t swn) {
+  auto fill_sched = [&](int par, uint64_t swn, bool dry) {
     if (lane < nblk) {
       int bo = lane;
       uint32_t mk = 0;
-      if (p.order == kBlockRandom) {
+      if (p.order == kBlockRandom && !dry) {
         const FeistelKey fk = block_order_key((uint32_t)nblk, swn, p.k0, p.k1);
         bo = (int)feistel_perm(fk, (uint32_t)lane);
         mk = block_mask(swn, (uint32_t)lane, p.k0, p.k1);
@@ -1020,18 +1021,62 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       L.msk[par][lane] = (uint8_t)mk;
     }
   };
+  // Reads are volatile so that each use reloads the entry (a short LDS read)
+  // instead of keeping it live in scalar registers across the block's work.
   auto sched_block = [&](int par, int step) {
-    return __builtin_amdgcn_readfirstlane((int)L.ord[par][step]);
+    return __builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.ord[par][step]);
   };
   auto sched_mask = [&](int par, int c) {
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)L.msk[par][c]);
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.msk[par][c]);
+  };
+
+  // Passes over the blocks.  Every `refresh` sweeps (p.refresh > 0 and
+  // S > refresh) a dry pass recomputes the energy: the tracked energy adds
+  // each sweep's accepted changes (simulation.py:157-158), and each change
+  // carries the f32 rounding of the matrix-product fields, which
+  // random-walks over millions of flips (the reference recomputes each field
+  // from its coupling row).  A dry pass runs the pipeline over every block
+  // with the proposals switched off, so each block's fields are the full
+  // products F = J s of the current state (with the couplings the products
+  // use); wave 0 sums E = s.(F/2 + h) in f64 and resets the tracked energy
+  // to it plus the offset the caller's initial energy had from the dry pass
+  // before the first sweep (zero unless the caller passed other energies).
+  // Integer couplings are exact either way.  Passes: a dry pass, then
+  // `refresh` sweeps and a dry pass, repeated; no dry pass after the last
+  // sweep.  pass_info gives a pass's kind and its sweep (for a dry pass the
+  // sweep that follows it).
+  const int kre = p.refresh;
+  const bool rec = kre > 0 && p.S > kre;
+  const int npass = rec ? p.S + 1 + (p.S - 1) / kre : p.S;
+  auto pass_info = [&](int ps, int& s_out) -> bool {
+    if (!rec) {
+      s_out = ps;
+      return false;
+    }
+    if (ps == 0) {
+      s_out = 0;
+      return true;
+    }
+    const int qq = ps - 1, bq = qq / (kre + 1), rq = qq - bq * (kre + 1);
+    if (rq == kre) {
+      s_out = (bq + 1) * kre;
+      return true;
+    }
+    s_out = bq * kre + rq;
+    return false;
   };
 
   // ---- prologue: the fields of the first block in full, its brackets and spins
-  if (w == 4) fill_sched(0, p.sweep_offset);
-  __syncthreads();  // wave 0's betas (L.rb), the zeroed L.corr and the schedule are visible
-  const uint64_t sw0 = p.sweep_offset;
   {
+    int s0;
+    const bool dry0 = pass_info(0, s0);
+    if (w == 4) fill_sched(0, (uint64_t)s0 + p.sweep_offset, dry0);
+  }
+  __syncthreads();  // wave 0's betas (L.rb), the zeroed L.corr and the schedule are visible
+  {
+    int s0;
+    (void)pass_info(0, s0);
+    const uint64_t sw0 = (uint64_t)s0 + p.sweep_offset;
     const int cf = sched_block(0, 0), cs = sched_block(0, 1);
     const uint32_t mf = sched_mask(0, cf), ms = sched_mask(0, cs);
     if (mw) {
@@ -1051,17 +1096,20 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   // rsb (zero otherwise); fsc is a power of two, so the fma rounds like the add
   const bool qsum = !I8 && p.ptype == 1;
   const float fsc = qsum ? 0.5f : 1.f;
+  double eoff = 0.0;  // wave 0: caller's initial energy minus its recomputation
 
   uint32_t t = 0;                               // running block counter
   // The two roles run separate copies of the block loop (so that each keeps
   // only its own state in registers); both pass the same barriers: one after
   // the block's work (B2), two more at the end of a sweep, one per block (B3).
   if (w == 0) {
-    for (int s = 0; s < p.S; ++s) {
+    for (int ps = 0; ps < npass; ++ps) {
+      int s;
+      const bool dry = pass_info(ps, s);
       const uint64_t sw = (uint64_t)s + p.sweep_offset;
-      const int sp = s & 1;
+      const int sp = ps & 1;
       const float beta = act ? L.rb[q] : 0.f;
-      double dsum = 0.0;
+      double dsum = 0.0, eacc = 0.0;
       uint32_t nacc = 0;
       for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1;
@@ -1069,49 +1117,68 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         const int c = sched_block(sp, tb);
         const uint32_t xm = sched_mask(sp, c);
         const int pos0 = 32 * c;
+        if (dry) {
+          // ---- dry pass: f64 energy of the block's rows from its full fields
+          const uint32_t bits = L.bb[pb][q];
+          for (int i = 0; i < kB; ++i) {
+            if (pos0 + i < n) {
+              const int o = fld(q, i);
+              double f = (double)L.part[pb][0][o];
+              for (int mm = 1; mm < MW; ++mm) f += (double)L.part[pb][mm][o];
+              f += (double)L.corr[o];
+              if (qsum) f = 0.5 * (f + (double)L.rsb[pb][i]);
+              const double si = ((bits >> i) & 1u) ? 1.0 : (p.ptype == 0 ? -1.0 : 0.0);
+              eacc += si * (0.5 * f + (double)L.hb[pb][i]);
+            }
+          }
+          L.bnew[q] = bits;
+          __syncthreads();  // B2
+          if (tb + 1 == nblk && act) {
+            if (ps == 0)
+              eoff = (double)E - eacc;
+            else
+              E = (float)(eacc + eoff);
+          }
+          __syncthreads();  // B3
+          continue;
+        }
         // ---- proposals of block c, one lane per replica
         float fp[kB];
         {
           const int xg = (int)(xm >> 2);
+          // visit group r4 holds block rows 4 (r4 ^ xg) + (e ^ (xm & 3)); the
+          // in-group part is a compile-time permutation per uniform branch
+          auto load_fields = [&](auto pc) __attribute__((always_inline)) {
+            constexpr int P = decltype(pc)::value;
 #pragma unroll
-          for (int r4 = 0; r4 < kB / 4; ++r4) {
-            // visit group r4 holds block rows 4 (r4 ^ xg) + (e ^ (xm & 3))
-            const int gr = r4 ^ xg;
-            const int o = fld(q, 4 * gr);
-            float4 sm = *reinterpret_cast<const float4*>(&L.part[pb][0][o]);
+            for (int r4 = 0; r4 < kB / 4; ++r4) {
+              const int gr = r4 ^ xg;
+              const int o = fld(q, 4 * gr);
+              float4 sm = *reinterpret_cast<const float4*>(&L.part[pb][0][o]);
 #pragma unroll
-            for (int mm = 1; mm < MW; ++mm) {
-              const float4 a = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
-              sm.x = sm.x + a.x;
-              sm.y = sm.y + a.y;
-              sm.z = sm.z + a.z;
-              sm.w = sm.w + a.w;
+              for (int mm = 1; mm < MW; ++mm) {
+                const float4 a = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
+                sm.x = sm.x + a.x;
+                sm.y = sm.y + a.y;
+                sm.z = sm.z + a.z;
+                sm.w = sm.w + a.w;
+              }
+              const float4 cr = *reinterpret_cast<const float4*>(&L.corr[o]);
+              const float4 h4 = *reinterpret_cast<const float4*>(&L.hb[pb][4 * gr]);
+              const float4 s4 = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
+              const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y, sm.z + cr.z + s4.z,
+                                   sm.w + cr.w + s4.w};
+              const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
+#pragma unroll
+              for (int e = 0; e < 4; ++e)
+                fp[4 * r4 + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
             }
-            const float4 cr = *reinterpret_cast<const float4*>(&L.corr[o]);
-            const float4 h4 = *reinterpret_cast<const float4*>(&L.hb[pb][4 * gr]);
-            const float4 s4 = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
-            const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y, sm.z + cr.z + s4.z,
-                                 sm.w + cr.w + s4.w};
-            const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
-            float f4[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) f4[e] = __builtin_fmaf(fsc, vs[e], hv[e]);
-            if (xm & 1u) {
-              const float a0 = f4[0], a2 = f4[2];
-              f4[0] = f4[1];
-              f4[1] = a0;
-              f4[2] = f4[3];
-              f4[3] = a2;
-            }
-            if (xm & 2u) {
-              const float a0 = f4[0], a1 = f4[1];
-              f4[0] = f4[2];
-              f4[1] = f4[3];
-              f4[2] = a0;
-              f4[3] = a1;
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e) fp[4 * r4 + e] = f4[e];
+          };
+          switch (xm & 3u) {
+            case 0: load_fields(std::integral_constant<int, 0>{}); break;
+            case 1: load_fields(std::integral_constant<int, 1>{}); break;
+            case 2: load_fields(std::integral_constant<int, 2>{}); break;
+            default: load_fields(std::integral_constant<int, 3>{}); break;
           }
         }
         // spins in visit order (bit k = block row k ^ xm)
@@ -1268,29 +1335,36 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       }
     }
   } else {
-    for (int s = 0; s < p.S; ++s) {
+    for (int ps = 0; ps < npass; ++ps) {
+      int s, s_next;
+      const bool dry = pass_info(ps, s);
+      const bool dry_next = pass_info(ps + 1, s_next);
       const uint64_t sw = (uint64_t)s + p.sweep_offset;
-      const int sp = s & 1;
+      const uint64_t sw_next = (uint64_t)s_next + p.sweep_offset;
+      const int sp = ps & 1;
       for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1, pn = pb ^ 1;
         (void)pb;
-        // the schedule of the next sweep, needed from step nblk - 2 on
-        if (w == 4 && tb == 0) fill_sched(sp ^ 1, sw + 1);
-        // block c of this step; c1 and c2 the next two (into the next sweep)
+        // the schedule of the next pass, needed from step nblk - 2 on
+        if (w == 4 && tb == 0) fill_sched(sp ^ 1, sw_next, dry_next);
+        // block c of this step; c1 and c2 the next two (into the next
+        // sweep), each read from the schedule where it is used
         const bool last = tb + 1 == nblk;
-        const int c = sched_block(sp, tb);
         const int p1 = last ? sp ^ 1 : sp, p2 = tb + 2 < nblk ? sp : sp ^ 1;
-        const int c1 = sched_block(p1, last ? 0 : tb + 1);
-        const int c2 = sched_block(p2, tb + 2 < nblk ? tb + 2 : tb + 2 - nblk);
-        const uint32_t m1 = sched_mask(p1, c1);
+        const int i1 = last ? 0 : tb + 1, i2 = tb + 2 < nblk ? tb + 2 : tb + 2 - nblk;
         // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
         if (mw) {
           publish_block(pn);
-          compute_part(c1, c, L.part[pn][m]);
+          compute_part(sched_block(p1, i1), sched_block(sp, tb), L.part[pn][m]);
+          const int c2 = sched_block(p2, i2);
           prefetch_block(c2, sched_mask(p2, c2));
         }
-        if ((mw || w == 4) && !last) brackets(c1, m1, sw, pn);
+        if ((mw || w == 4) && !last && !dry) {
+          const int c1 = sched_block(p1, i1);
+          brackets(c1, sched_mask(p1, c1), sw, pn);
+        }
         __syncthreads();  // B2
+        const int c = sched_block(sp, tb), c1 = sched_block(p1, i1);
         if (mw && wl == owner_of(c)) {
           // ---- the owners of block c take its new spins and add J[c1, c] . S_c
           const int j = c - c0;
@@ -1302,14 +1376,18 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         }
         if (mw && wl == owner_of(c1)) publish_bits(c1, pn);
         if (last) {
-          __syncthreads();
-          if (mw && p.best_sem == 0 && actt && L.flag[qt])
-            store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_t);
-          __syncthreads();
-          // owners copy their part of a new global best
-          if (mw && p.best_sem == 1 && actt && L.src[gt] == qt - gt * R)
-            store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gt) * n);
-          if (mw || w == 4) brackets(c1, m1, sw + 1, pn);  // next sweep's first block, new betas
+          if (!dry) {
+            __syncthreads();
+            if (mw && p.best_sem == 0 && actt && L.flag[qt])
+              store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_t);
+            __syncthreads();
+            // owners copy their part of a new global best
+            if (mw && p.best_sem == 1 && actt && L.src[gt] == qt - gt * R)
+              store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype,
+                                  p.best_state + (size_t)(read0 + gt) * n);
+          }
+          // the next pass's first block (after the exchanges: new betas)
+          if ((mw || w == 4) && !dry_next) brackets(c1, sched_mask(p1, c1), sw_next, pn);
         }
         __syncthreads();  // B3
       }
@@ -1584,6 +1662,7 @@ hipError_t launch_blocked(const BlockedProblem& b, const BlockedPlan& pl, const 
   p.Jt = pl.inexact ? b.jtl : (const float*)dJt;
   p.h = (const float*)dh;
   p.order = order;
+  p.refresh = 64;
   p.rowsum = (!pl.i8 && ptype == 1) ? b.rowsum : nullptr;
   p.betas = (const float*)betas;
   p.bsort = bsort;

@@ -165,3 +165,26 @@ def test_blocked_float_qubo(kind, order, form):
     assert rel.max() < 1e-5, rel.max()
     b64 = oracle.energy_f64(J, h, got["best_state"])
     assert np.all(np.abs(got["best_energy"] - b64) <= 1e-5 * np.maximum(1.0, np.abs(b64)))
+
+
+@pytest.mark.parametrize("order", [0, 3])
+@pytest.mark.parametrize("ptype", [0, 1])
+def test_blocked_energy_recompute_keeps_caller_offset(order, ptype):
+    """The MFMA form recomputes the energy in dry passes every 64 sweeps and
+    resets the tracked energy to it plus the offset the caller's initial
+    energy had from it.  With integer couplings the run stays bit-exact
+    against the oracle, whose tracked energy simply carries the offset
+    (pysa/simulation.py:157-158), over several recomputations."""
+    rng = np.random.default_rng(808 + order + 2 * ptype)
+    n, R, reads, S = 130, 16, 4, 200
+    J, h = int_problem(n, rng, -5, 5, 2)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    betas = (1 / np.geomspace(0.8, 4.0, R)).astype(np.float32)
+    st = random_spins((reads, R, n), rng, ptype)
+    en = energies_of(J, h, st, ptype, np.float32) + np.float32(7.0)  # caller's offset
+    kw = dict(order=order, seed=31)
+    ref = oracle.metropolis(J, h, ptype, betas, st, en, S, nthreads=8, **kw)
+    got = engine_metropolis(J, h, ptype, betas, st, en, S, form=_lib.FORM_BLOCKED, **kw)
+    assert_same(got, ref)
+    e64 = oracle.energy_f64(J, h, got["states"].reshape(-1, n)).reshape(reads, R)
+    assert np.array_equal(got["energies"].astype(np.float64), e64 + 7.0)

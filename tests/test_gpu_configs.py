@@ -106,3 +106,25 @@ def test_c3_wishart_full_n(strategy):
         # within ~1e-7 of the f64 energy.
         assert rel.max() < 1e-5, (form, rel.max())
         assert errs[(form, "best")].max() < 1e-5, form
+
+
+def test_c3_energy_error_full_length():
+    """The benchmark's full length, 5000 sweeps of the planted Wishart
+    instance with update_strategy='random': tracked and best energies
+    within 1e-5 of f64 for both forms (the plain form resets its energy
+    with each f64 field refresh, the MFMA form in a dry recomputation pass
+    every 64 sweeps)."""
+    P, egs = instances.planted_wishart(1024, 0.5, 1003)
+    J, h = split(P)
+    reads, R, S = 4, 32, 5000
+    rng = np.random.default_rng(2003)
+    st = (2 * rng.integers(0, 2, (reads, R, 1024)) - 1).astype(np.int8)
+    en = f64_energies(J, h, st).astype(np.float32)
+    betas = ladder(R, 0.3, 1.5)
+    for form, order in ((_lib.FORM_PLAIN, 1), (_lib.FORM_BLOCKED, 3)):
+        got = engine_metropolis(J, h, 0, betas, st, en, S, order=order, seed=2003, form=form)
+        e64 = f64_energies(J, h, got["states"])
+        rel = np.abs(got["energies"] - e64) / np.abs(e64)
+        assert rel.max() < 1e-5, (form, rel.max())
+        be64 = f64_energies(J, h, got["best_state"][:, None, :])[:, 0]
+        assert np.all(np.abs(got["best_energy"] - be64) / np.abs(be64) < 1e-5), form

@@ -407,3 +407,38 @@ def test_batched_oracle_matches_single_instance_oracle():
         for k in ("states", "energies", "beta_idx", "best_state", "best_energy", "best_sweeps",
                   "accepted"):
             assert np.array_equal(got[k].reshape(-1), want[k].reshape(-1)), k
+
+
+def test_pt_log_omega_estimator_expectation():
+    """The reference's PT estimate of log Z (get_log_omega per sweep,
+    pysa/ais.py:79-112, then omegas_to_partition) weighs each rung by the
+    state sampled at the higher beta.  PT's stationary law is the product of
+    the slots' Boltzmann laws, so E[omega] = prod_r Z(2 b_r - b_(r-1)) / Z(b_r)
+    over the ascending ladder, a closed form at n = 12 by enumeration; the
+    oracle's estimate (400 reads x 250 sweeps after burn-in) matches it."""
+    from scipy.special import logsumexp
+    from pysa_amd import ais as pais
+
+    rng = np.random.default_rng(5)
+    n = 12
+    c = np.triu(rng.normal(0, 0.5, (n, n)), 1)
+    J = (c + c.T).astype(np.float32)
+    h = rng.normal(0, 0.3, n).astype(np.float32)
+    allE = energies_np(J, h, all_states(n, 0))
+
+    def logZ(b):
+        return logsumexp(-b * allE)
+
+    R = 24
+    ratio = np.exp(1 / (R - 2) * np.log(20.0 / 0.5))
+    betas = np.append([1 / (0.5 * ratio**k) for k in range(R - 1)], 0.0).astype(np.float32)
+    b = np.sort(betas.astype(np.float64))
+    pred = n * np.log(2) + sum(logZ(2 * b[r] - b[r - 1]) - logZ(b[r]) for r in range(1, R))
+    reads, S = 400, 300
+    st = 2 * rng.integers(0, 2, (reads, R, n)) - 1
+    out = run(J, h, 0, betas, st, S, order=0, seed=3, log_omega=True, nthreads=8)
+    lz = np.array([pais.omegas_to_partition(out["log_omegas"][r][50:], n * np.log(2))
+                   for r in range(reads)])
+    comb = logsumexp(lz) - np.log(reads)
+    assert abs(comb - pred) < 0.02, (comb, pred)
+    assert abs(pred - logZ(b[-1])) > 0.5  # the estimator is not log Z itself
