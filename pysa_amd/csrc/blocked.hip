@@ -746,9 +746,9 @@ struct __align__(16) PipeLds {
   float lo[2][kB][64];        // acceptance brackets [parity][row][replica]
   float hi[2][kB][64];
   uint32_t bb[2][64];         // block spins before the block (bit r = row r)
-  uint32_t bnew[64];          // block spins after the block
+  uint32_t bnew[2][64];       // block spins after the block (by block parity)
   float sfp[64 * 33];         // exact rerun staging
-  // visit schedule by sweep parity: block at step t, in-block XOR mask of block b
+  // visit schedule by pass parity: block at step t and its in-block XOR mask
   uint8_t ord[2][64];
   uint8_t msk[2][64];
   float E[64];
@@ -759,7 +759,10 @@ struct __align__(16) PipeLds {
   double lu[64];
 };
 
-template <int NP, bool I8, int NW, int MW>
+// BR: block-random visit order (schedule tables in LDS); otherwise blocks are
+// visited in order with no in-block shuffle, and every schedule entry is
+// plain arithmetic on the step.
+template <int NP, bool I8, int NW, int MW, bool BR>
 __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p) {
   __shared__ PipeLds<MW> L;
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tid = threadIdx.x;
@@ -827,7 +830,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   constexpr int NST = I8 ? 1 : 2;
   constexpr int NJB = (kB * kB + 128 * MW - 1) / (128 * MW);
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
-  Frag ring[PF][NST][NP], fcorr[NST][NP];
+  Frag ring[PF][NST][NP];
   float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
   uint32_t jb_ok = 0;  // bit e2: jb_reg[e2] is a coupling; bit 31: h_reg / rs_reg are
   auto frag_ptr = [&](int c, int j, int st, int pc) -> const Frag* {
@@ -924,7 +927,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   };
   // Fields from the C layout (register v: row 8(v/4) + 4h + v%4 of column
   // lane%32) into [replica][row], four consecutive rows per 16-byte store.
-  auto write_fields = [&](float* dst, const f32x16& acc, const i32x16& iacc) {
+  auto write_fields_col = [&](float* dst, const f32x16& acc, const i32x16& iacc, int qcol) {
 #pragma unroll
     for (int g4 = 0; g4 < 4; ++g4) {
       const int i = 8 * g4 + 4 * h;
@@ -934,12 +937,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
                         (float)iacc[4 * g4 + 3]);
       else
         x = make_float4(acc[4 * g4], acc[4 * g4 + 1], acc[4 * g4 + 2], acc[4 * g4 + 3]);
-      *reinterpret_cast<float4*>(&dst[fld(qt, i)]) = x;
+      *reinterpret_cast<float4*>(&dst[fld(qcol, i)]) = x;
     }
   };
+  auto write_fields = [&](float* dst, const f32x16& acc, const i32x16& iacc) {
+    write_fields_col(dst, acc, iacc, qt);
+  };
   // Partial fields of row block cr over this wave's K range, skipping K block
-  // ex (its fragments are kept in fcorr for the correction); ring holds the
-  // first PF stages of row block cr on entry.
+  // ex (its product is added by the proposal wave once the block's spins are
+  // final); ring holds the first PF stages of row block cr on entry.
   auto compute_part = [&](int cr, int ex, float* dst) {
     f32x16 acc = {};
     i32x16 iacc = {};
@@ -959,32 +965,21 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         // fragments: the load is issued after the products that read them);
         // slots past the 2 NW spin halves exist only when PF does not divide
         // 2 NW and are never live
-        if (j < 2 * NW && u < jrem) {
-          if (j == jx) {
-#pragma unroll
-            for (int st = 0; st < NST; ++st)
-#pragma unroll
-              for (int pc = 0; pc < NP; ++pc) fcorr[st][pc] = ring[u][st][pc];
-          } else {
-            mfma_step(ring[u], get_half<NW>(sa, j), acc, iacc);
-          }
-        }
+        if (j < 2 * NW && u < jrem && j != jx) mfma_step(ring[u], get_half<NW>(sa, j), acc, iacc);
         load_stage(cr, j + PF, ring[u]);
       }
     }
     write_fields(dst, acc, iacc);
   };
   // Acceptance brackets of row block cr (in-block mask xm) for sweep counter
-  // sw, stored by visit index: this matrix wave's share of the rows, lane =
-  // replica (beta from L.rb).
+  // sw, stored by visit index, lane = replica (beta from L.rb).  Wave 4
+  // computes them all: it shares its SIMD with the proposal wave, whose VALU
+  // chain leaves issue slots, while the matrix SIMDs are the critical path.
   auto brackets = [&](int cr, uint32_t xm, uint64_t sw, int pb) {
     float cb, cz;
     bracket_factors(L.rb[q], cb, cz);
     const int pos0 = 32 * cr;
-    // groups of 4 rows (one Philox block each): one per matrix wave, the
-    // rest to wave 4 (which shares a SIMD with the proposal wave only)
-    const int gstart = mw ? mi : 2 * MW, gstep = mw ? kB / 4 : 1;
-    for (int g4 = gstart; g4 < kB / 4; g4 += gstep) {
+    for (int g4 = 0; g4 < kB / 4; ++g4) {  // groups of 4 rows, one Philox block each
       const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
       const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
@@ -1007,7 +1002,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   };
 
   // Visit schedule of sweep counter swn into parity par (wave 4, one lane
-  // per block): the block visited at step b and the in-block mask of block b.
+  // per step): the block visited at step t and that block's in-block mask.
   auto fill_sched = [&](int par, uint64_t swn, bool dry) {
     if (lane < nblk) {
       int bo = lane;
@@ -1015,7 +1010,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       if (p.order == kBlockRandom && !dry) {
         const FeistelKey fk = block_order_key((uint32_t)nblk, swn, p.k0, p.k1);
         bo = (int)feistel_perm(fk, (uint32_t)lane);
-        mk = block_mask(swn, (uint32_t)lane, p.k0, p.k1);
+        mk = block_mask(swn, (uint32_t)bo, p.k0, p.k1);
       }
       L.ord[par][lane] = (uint8_t)bo;
       L.msk[par][lane] = (uint8_t)mk;
@@ -1024,10 +1019,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   // Reads are volatile so that each use reloads the entry (a short LDS read)
   // instead of keeping it live in scalar registers across the block's work.
   auto sched_block = [&](int par, int step) {
+    if (!BR) return step;
     return __builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.ord[par][step]);
   };
-  auto sched_mask = [&](int par, int c) {
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.msk[par][c]);
+  auto sched_mask = [&](int par, int step) {
+    if (!BR) return 0u;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.msk[par][step]);
   };
 
   // Passes over the blocks.  Every `refresh` sweeps (p.refresh > 0 and
@@ -1070,7 +1067,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   {
     int s0;
     const bool dry0 = pass_info(0, s0);
-    if (w == 4) fill_sched(0, (uint64_t)s0 + p.sweep_offset, dry0);
+    if (BR && w == 4) fill_sched(0, (uint64_t)s0 + p.sweep_offset, dry0);
   }
   __syncthreads();  // wave 0's betas (L.rb), the zeroed L.corr and the schedule are visible
   {
@@ -1078,7 +1075,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     (void)pass_info(0, s0);
     const uint64_t sw0 = (uint64_t)s0 + p.sweep_offset;
     const int cf = sched_block(0, 0), cs = sched_block(0, 1);
-    const uint32_t mf = sched_mask(0, cf), ms = sched_mask(0, cs);
+    const uint32_t mf = sched_mask(0, 0), ms = sched_mask(0, 1);
     if (mw) {
       prefetch_block(cf, mf);
       publish_block(0);
@@ -1086,7 +1083,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       prefetch_block(cs, ms);
       if (wl == owner_of(cf)) publish_bits(cf, 0);
     }
-    if (mw || w == 4) brackets(cf, mf, sw0, 0);
+    if (w == 4) brackets(cf, mf, sw0, 0);
   }
   __syncthreads();
 
@@ -1098,11 +1095,44 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   const float fsc = qsum ? 0.5f : 1.f;
   double eoff = 0.0;  // wave 0: caller's initial energy minus its recomputation
 
+  // The correction J[c1, c] . S_c of the next block c1 by the proposal wave,
+  // once block c's spins are final (on SIMD 0's matrix core, idle otherwise):
+  // bits = block c's spins of replica `lane` in block-row order; the A
+  // fragments of row block c1, K block c were loaded into fa at the block's
+  // start.  Result to L.corr (read by this wave at the next block's start).
+  typedef Frag FragSet[NST][NP];
+  auto load_corr_frags = [&](int c1, int c, FragSet& fa) {
+    const Frag* fb = reinterpret_cast<const Frag*>(p.frag);
+#pragma unroll
+    for (int st = 0; st < NST; ++st)
+#pragma unroll
+      for (int pc = 0; pc < NP; ++pc) {
+        const size_t f = I8 ? (size_t)c1 * nblk + c
+                            : ((size_t)c1 * (2 * nblk) + 2 * c + st) * NP + pc;
+        fa[st][pc] = fb[f * 64 + lane];
+      }
+  };
+  auto corr_products = [&](const FragSet& fa, uint32_t bits) {
+#pragma unroll
+    for (int tl = 0; tl < 2; ++tl) {
+      // column lane % 32 of tile tl is replica 32 tl + lane % 32
+      const uint32_t bt = (uint32_t)__shfl((int)bits, 32 * tl + (lane & 31), 64);
+      f32x16 a = {};
+      i32x16 ia = {};
+      mfma_step(fa, half_of<I8>(bt, h), a, ia);
+      write_fields_col(L.corr, a, ia, 32 * tl + (lane & 31));
+    }
+  };
+
   uint32_t t = 0;                               // running block counter
   // The two roles run separate copies of the block loop (so that each keeps
   // only its own state in registers); both pass the same barriers: one after
-  // the block's work (B2), two more at the end of a sweep, one per block (B3).
+  // the block's work (B), and at the end of a pass one more (three after a
+  // sweep).
   if (w == 0) {
+    // schedule entries of the current step, each read one block ahead
+    int c_nx = sched_block(0, 0);
+    uint32_t xm_nx = sched_mask(0, 0);
     for (int ps = 0; ps < npass; ++ps) {
       int s;
       const bool dry = pass_info(ps, s);
@@ -1113,10 +1143,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       uint32_t nacc = 0;
       for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1;
-        // block c of this step, its visit k being block row k ^ xm
-        const int c = sched_block(sp, tb);
-        const uint32_t xm = sched_mask(sp, c);
+        // block c of this step, its visit k being block row k ^ xm; c1 the next
+        const int c = c_nx;
+        const uint32_t xm = xm_nx;
         const int pos0 = 32 * c;
+        const bool lastb = tb + 1 == nblk;
+        const int c1 = sched_block(lastb ? sp ^ 1 : sp, lastb ? 0 : tb + 1);
+        xm_nx = sched_mask(lastb ? sp ^ 1 : sp, lastb ? 0 : tb + 1);
+        c_nx = c1;
+        FragSet fcorr;
+        load_corr_frags(c1, c, fcorr);
         if (dry) {
           // ---- dry pass: f64 energy of the block's rows from its full fields
           const uint32_t bits = L.bb[pb][q];
@@ -1131,15 +1167,19 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               eacc += si * (0.5 * f + (double)L.hb[pb][i]);
             }
           }
-          L.bnew[q] = bits;
-          __syncthreads();  // B2
-          if (tb + 1 == nblk && act) {
-            if (ps == 0)
-              eoff = (double)E - eacc;
-            else
-              E = (float)(eacc + eoff);
+          L.bnew[pb][q] = bits;
+          if (c1 == c) L.bb[pb ^ 1][q] = bits;  // the same block next (the owners skip it)
+          corr_products(fcorr, bits);
+          __syncthreads();  // B
+          if (lastb) {
+            if (act) {
+              if (ps == 0)
+                eoff = (double)E - eacc;
+              else
+                E = (float)(eacc + eoff);
+            }
+            __syncthreads();  // the next pass's first brackets
           }
-          __syncthreads();  // B3
           continue;
         }
         // ---- proposals of block c, one lane per replica
@@ -1264,11 +1304,18 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               sf[r] = __builtin_fmaf(L.jb[pb][k * kB + (r - k - 1)], dS, sf[r]);
           }
         }
-        L.bnew[q] = xor_permute_bits(mb, xm);  // back to block-row order
+        {
+          const uint32_t mbr = xor_permute_bits(mb, xm);  // back to block-row order
+          L.bnew[pb][q] = mbr;
+          // the same block next (at a pass boundary): its spins are these
+          // (the owners, whose copy is older, skip it)
+          if (c1 == c) L.bb[pb ^ 1][q] = mbr;
+          corr_products(fcorr, mbr);
+        }
         dsum += (double)dpart;
         nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
-        __syncthreads();  // B2
-        if (tb + 1 == nblk) {
+        __syncthreads();  // B
+        if (lastb) {
           // ---- end of sweep: energy (simulation.py:157-158), best states, PT
           acc += nacc;
           E = (float)((double)E + dsum);
@@ -1330,11 +1377,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             L.src[g] = src;
           }
           __syncthreads();
+          __syncthreads();  // the next pass's first brackets
         }
-        __syncthreads();  // B3
       }
     }
   } else {
+    // schedule entries carried from block to block: c (this step), c1 (the
+    // next) and its mask; c2 (the one after) is read each block
+    int c = sched_block(0, 0), c1 = sched_block(0, 1);
+    uint32_t m1 = sched_mask(0, 1);
     for (int ps = 0; ps < npass; ++ps) {
       int s, s_next;
       const bool dry = pass_info(ps, s);
@@ -1344,37 +1395,25 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       const int sp = ps & 1;
       for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1, pn = pb ^ 1;
-        (void)pb;
         // the schedule of the next pass, needed from step nblk - 2 on
-        if (w == 4 && tb == 0) fill_sched(sp ^ 1, sw_next, dry_next);
-        // block c of this step; c1 and c2 the next two (into the next
-        // sweep), each read from the schedule where it is used
+        if (BR && w == 4 && tb == 0) fill_sched(sp ^ 1, sw_next, dry_next);
         const bool last = tb + 1 == nblk;
-        const int p1 = last ? sp ^ 1 : sp, p2 = tb + 2 < nblk ? sp : sp ^ 1;
-        const int i1 = last ? 0 : tb + 1, i2 = tb + 2 < nblk ? tb + 2 : tb + 2 - nblk;
-        // ---- meanwhile: block c1's fields except K block c, its J / h and brackets
+        const int p2 = tb + 2 < nblk ? sp : sp ^ 1, i2 = tb + 2 < nblk ? tb + 2 : tb + 2 - nblk;
+        const int c2 = sched_block(p2, i2);
+        const uint32_t m2 = sched_mask(p2, i2);
+        // ---- meanwhile: block c1's spins, fields except K block c, J / h and brackets
         if (mw) {
           publish_block(pn);
-          compute_part(sched_block(p1, i1), sched_block(sp, tb), L.part[pn][m]);
-          const int c2 = sched_block(p2, i2);
-          prefetch_block(c2, sched_mask(p2, c2));
+          // c1's spins have not changed since its last visit unless c1 == c
+          // (a pass boundary), which the proposal wave publishes itself
+          if (wl == owner_of(c1) && c1 != c) publish_bits(c1, pn);
+          compute_part(c1, c, L.part[pn][m]);
+          prefetch_block(c2, m2);
         }
-        if ((mw || w == 4) && !last && !dry) {
-          const int c1 = sched_block(p1, i1);
-          brackets(c1, sched_mask(p1, c1), sw, pn);
-        }
-        __syncthreads();  // B2
-        const int c = sched_block(sp, tb), c1 = sched_block(p1, i1);
-        if (mw && wl == owner_of(c)) {
-          // ---- the owners of block c take its new spins and add J[c1, c] . S_c
-          const int j = c - c0;
-          set_half<NW>(sa, j, half_of<I8>(L.bnew[qt], h));
-          f32x16 acc = {};
-          i32x16 iacc = {};
-          mfma_step(fcorr, get_half<NW>(sa, j), acc, iacc);
-          write_fields(L.corr, acc, iacc);
-        }
-        if (mw && wl == owner_of(c1)) publish_bits(c1, pn);
+        if (w == 4 && !last && !dry) brackets(c1, m1, sw, pn);
+        __syncthreads();  // B
+        // ---- the owners of block c take its new spins
+        if (mw && wl == owner_of(c)) set_half<NW>(sa, c - c0, half_of<I8>(L.bnew[pb][qt], h));
         if (last) {
           if (!dry) {
             __syncthreads();
@@ -1387,9 +1426,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
                                   p.best_state + (size_t)(read0 + gt) * n);
           }
           // the next pass's first block (after the exchanges: new betas)
-          if ((mw || w == 4) && !dry_next) brackets(c1, sched_mask(p1, c1), sw_next, pn);
+          if (w == 4 && !dry_next) brackets(c1, m1, sw_next, pn);
+          __syncthreads();
         }
-        __syncthreads();  // B3
+        c = c1;
+        c1 = c2;
+        m1 = m2;
       }
     }
   }
@@ -1442,16 +1484,22 @@ hipError_t launch_kw(const BParams& p, int nw, int groups, hipStream_t st) {
 }
 
 
-template <int NP, bool I8>
-hipError_t launch_pipe(const BParams& p, int nw, int groups, hipStream_t st) {
+template <int NP, bool I8, bool BR>
+hipError_t launch_pipe_o(const BParams& p, int nw, int groups, hipStream_t st) {
   dim3 grid(groups), block(128 * (kPipeMW + 1));
   switch (nw) {
-    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW>), grid, block, 0, st, p); break;
-    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW>), grid, block, 0, st, p); break;
-    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW>), grid, block, 0, st, p); break;
+    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW, BR>), grid, block, 0, st, p); break;
+    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW, BR>), grid, block, 0, st, p); break;
+    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW, BR>), grid, block, 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int NP, bool I8>
+hipError_t launch_pipe(const BParams& p, int nw, int groups, hipStream_t st) {
+  return p.order == kBlockRandom ? launch_pipe_o<NP, I8, true>(p, nw, groups, st)
+                                 : launch_pipe_o<NP, I8, false>(p, nw, groups, st);
 }
 
 template <int NP, bool I8>
@@ -1662,7 +1710,10 @@ hipError_t launch_blocked(const BlockedProblem& b, const BlockedPlan& pl, const 
   p.Jt = pl.inexact ? b.jtl : (const float*)dJt;
   p.h = (const float*)dh;
   p.order = order;
-  p.refresh = 64;
+  // PYSA_AMD_ENERGY_REFRESH=<sweeps> overrides the interval for every
+  // problem (0 = never), a switch for tests and comparisons
+  const char* ev = getenv("PYSA_AMD_ENERGY_REFRESH");
+  p.refresh = ev && ev[0] ? atoi(ev) : (b.exact_sums == 1 ? 0 : 64);
   p.rowsum = (!pl.i8 && ptype == 1) ? b.rowsum : nullptr;
   p.betas = (const float*)betas;
   p.bsort = bsort;
@@ -1761,6 +1812,24 @@ int noalloc_scratch(void** ptr, size_t* bytes, std::string* why) {
   return 0;
 }
 
+// Whether J and h are integers small enough that every field and energy sum
+// (at most sum_j |J_ij| + |h_i| in magnitude) is exact in f32: the tracked
+// energy then never drifts and the recomputation passes are skipped.
+bool integer_sums_exact(const float* J, const float* h, int n) {
+  double worst = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (!(fabsf(h[i]) < 16777216.f) || h[i] != (float)(int64_t)h[i]) return false;
+    double row = fabs((double)h[i]);
+    for (int j = 0; j < n; ++j) {
+      const float v = J[(size_t)i * n + j];
+      if (!(fabsf(v) < 16777216.f) || v != (float)(int64_t)v) return false;
+      row += fabs((double)v);
+    }
+    worst = row > worst ? row : worst;
+  }
+  return worst < 16777216.0 / 4;  // margin for the QUBO row-sum form and 2 x dn
+}
+
 int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* dh, int n,
                 int ptype, int dtype, int reads, int R, int S, int order, int use_pt, int logw,
                 int best_sem, int pieces, uint64_t seed, uint64_t sweep_offset, const void* betas,
@@ -1773,12 +1842,14 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     return PYSA_AMD_ENOTSUP;
   }
   if (b.exact_kind < 0) {
-    std::vector<float> hJ((size_t)n * n);
-    if (hipMemcpy(hJ.data(), dJ, hJ.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+    std::vector<float> hJ((size_t)n * n), hh(n);
+    if (hipMemcpy(hJ.data(), dJ, hJ.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hh.data(), dh, hh.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
       *why = "reading couplings back";
       return PYSA_AMD_EHIP;
     }
     b.exact_kind = blocked_exact_pieces(hJ.data(), hJ.size());
+    b.exact_sums = integer_sums_exact(hJ.data(), hh.data(), n) ? 1 : 0;
   }
   const BlockedPlan pl = plan_blocked(b, n, reads, R, order, pieces);
   int rc = prepare_encoding(b, dJ, dJt, n, ptype, pl, st, why);

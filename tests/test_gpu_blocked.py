@@ -170,11 +170,21 @@ def test_blocked_float_qubo(kind, order, form):
 @pytest.mark.parametrize("order", [0, 3])
 @pytest.mark.parametrize("ptype", [0, 1])
 def test_blocked_energy_recompute_keeps_caller_offset(order, ptype):
-    """The MFMA form recomputes the energy in dry passes every 64 sweeps and
-    resets the tracked energy to it plus the offset the caller's initial
-    energy had from it.  With integer couplings the run stays bit-exact
+    """The MFMA form recomputes the energy in dry passes (every 64 sweeps for
+    float couplings; forced here every 30 sweeps with integer couplings,
+    which otherwise skip them) and resets the tracked energy to it plus the
+    offset the caller's initial energy had from it.  The run stays bit-exact
     against the oracle, whose tracked energy simply carries the offset
     (pysa/simulation.py:157-158), over several recomputations."""
+    import os
+    os.environ["PYSA_AMD_ENERGY_REFRESH"] = "30"
+    try:
+        _check_recompute_offset(order, ptype)
+    finally:
+        del os.environ["PYSA_AMD_ENERGY_REFRESH"]
+
+
+def _check_recompute_offset(order, ptype):
     rng = np.random.default_rng(808 + order + 2 * ptype)
     n, R, reads, S = 130, 16, 4, 200
     J, h = int_problem(n, rng, -5, 5, 2)
