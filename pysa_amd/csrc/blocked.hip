@@ -972,14 +972,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     write_fields(dst, acc, iacc);
   };
   // Acceptance brackets of row block cr (in-block mask xm) for sweep counter
-  // sw, stored by visit index, lane = replica (beta from L.rb).  Wave 4
-  // computes them all: it shares its SIMD with the proposal wave, whose VALU
-  // chain leaves issue slots, while the matrix SIMDs are the critical path.
+  // sw, stored by visit index, lane = replica (beta from L.rb): groups of 4
+  // rows (one Philox block each), one per matrix wave and the other two on
+  // wave 4, so that little of this work lands on the proposal wave's SIMD
+  // (the proposal chain is the critical path).
   auto brackets = [&](int cr, uint32_t xm, uint64_t sw, int pb) {
     float cb, cz;
     bracket_factors(L.rb[q], cb, cz);
     const int pos0 = 32 * cr;
-    for (int g4 = 0; g4 < kB / 4; ++g4) {  // groups of 4 rows, one Philox block each
+    const int gstart = mw ? mi : 2 * MW, gstep = mw ? kB / 4 : 1;
+    for (int g4 = gstart; g4 < kB / 4; g4 += gstep) {
       const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
       const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
 #pragma unroll
@@ -1083,7 +1085,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       prefetch_block(cs, ms);
       if (wl == owner_of(cf)) publish_bits(cf, 0);
     }
-    if (w == 4) brackets(cf, mf, sw0, 0);
+    if (mw || w == 4) brackets(cf, mf, sw0, 0);
   }
   __syncthreads();
 
@@ -1410,7 +1412,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           compute_part(c1, c, L.part[pn][m]);
           prefetch_block(c2, m2);
         }
-        if (w == 4 && !last && !dry) brackets(c1, m1, sw, pn);
+        if ((mw || w == 4) && !last && !dry) brackets(c1, m1, sw, pn);
         __syncthreads();  // B
         // ---- the owners of block c take its new spins
         if (mw && wl == owner_of(c)) set_half<NW>(sa, c - c0, half_of<I8>(L.bnew[pb][qt], h));
@@ -1426,7 +1428,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
                                   p.best_state + (size_t)(read0 + gt) * n);
           }
           // the next pass's first block (after the exchanges: new betas)
-          if (w == 4 && !dry_next) brackets(c1, m1, sw_next, pn);
+          if ((mw || w == 4) && !dry_next) brackets(c1, m1, sw_next, pn);
           __syncthreads();
         }
         c = c1;
