@@ -734,7 +734,13 @@ constexpr int kPipePF = 3;  // fragment stages in flight per matrix wave (slot r
 // A replica's 32 fields take 32 floats; its 16-byte row groups are XOR-
 // swizzled by the replica index so that 8 lanes of a 16-byte access hit all
 // 32 banks.
+__device__ __forceinline__ void swap2(float& a, float& b) {
+  const float t = a;
+  a = b;
+  b = t;
+}
 __device__ __forceinline__ int fld(int q, int r) { return q * 32 + 4 * ((r >> 2) ^ (q & 7)) + (r & 3); }
+constexpr int kHL = 36;
 template <int MW>
 struct __align__(16) PipeLds {
   float part[2][MW][64 * 32];  // [parity][matrix wave][fld(replica, row)]
@@ -743,8 +749,10 @@ struct __align__(16) PipeLds {
                               // J^T[pos0+i][pos0+i+1+t] at i*32 + t (t < 31 - i), else 0
   float hb[2][kB];
   float rsb[2][kB];
-  float lo[2][kB][64];        // acceptance brackets [parity][row][replica]
-  float hi[2][kB][64];
+  // acceptance brackets [parity][replica][visit index], rows padded to
+  // kHL floats so that 16-byte reads of consecutive replicas hit distinct banks
+  float lo[2][64][kHL];
+  float hi[2][64][kHL];
   uint32_t bb[2][64];         // block spins before the block (bit r = row r)
   uint32_t bnew[2][64];       // block spins after the block (by block parity)
   float sfp[64 * 33];         // exact rerun staging
@@ -984,15 +992,25 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     for (int g4 = gstart; g4 < kB / 4; g4 += gstep) {
       const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
       const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
+      float lo[4], hi[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = 4 * g4 + e, pos = pos0 + r;
-        float lo, hi;
-        accept_bounds_f(wd[e], cb, cz, lo, hi);
-        if (pos >= n || !act) lo = hi = INFINITY;  // padding and idle lanes: never taken
-        L.lo[pb][r ^ (int)xm][q] = lo;
-        L.hi[pb][r ^ (int)xm][q] = hi;
+        accept_bounds_f(wd[e], cb, cz, lo[e], hi[e]);
+        if (pos0 + 4 * g4 + e >= n || !act) lo[e] = hi[e] = INFINITY;  // padding, idle lanes
       }
+      // rows 4 g4 + e are visits 4 (g4 ^ xm / 4) + (e ^ xm % 4): one 16-byte
+      // store each, its entries exchanged by the uniform low bits of xm
+      if (xm & 1u) {
+        swap2(lo[0], lo[1]), swap2(lo[2], lo[3]);
+        swap2(hi[0], hi[1]), swap2(hi[2], hi[3]);
+      }
+      if (xm & 2u) {
+        swap2(lo[0], lo[2]), swap2(lo[1], lo[3]);
+        swap2(hi[0], hi[2]), swap2(hi[1], hi[3]);
+      }
+      const int v4 = 4 * (g4 ^ (int)(xm >> 2));
+      *reinterpret_cast<float4*>(&L.lo[pb][q][v4]) = make_float4(lo[0], lo[1], lo[2], lo[3]);
+      *reinterpret_cast<float4*>(&L.hi[pb][q][v4]) = make_float4(hi[0], hi[1], hi[2], hi[3]);
     }
   };
   auto owner_of = [&](int c) { return 1 + c / kb; };  // wave owning K block c
@@ -1185,6 +1203,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           continue;
         }
         // ---- proposals of block c, one lane per replica
+        // brackets of the block's 32 visits, read first (their latency
+        // overlaps the field sums and they leave the chain free of LDS waits)
+        float hib[kB], lob[kB];
+#pragma unroll
+        for (int t4 = 0; t4 < kB / 4; ++t4) {
+          const float4 a = *reinterpret_cast<const float4*>(&L.hi[pb][q][4 * t4]);
+          const float4 b = *reinterpret_cast<const float4*>(&L.lo[pb][q][4 * t4]);
+          hib[4 * t4] = a.x, hib[4 * t4 + 1] = a.y, hib[4 * t4 + 2] = a.z, hib[4 * t4 + 3] = a.w;
+          lob[4 * t4] = b.x, lob[4 * t4 + 1] = b.y, lob[4 * t4 + 2] = b.z, lob[4 * t4 + 3] = b.w;
+        }
         float fp[kB];
         {
           const int xg = (int)(xm >> 2);
@@ -1250,7 +1278,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
           const float sg = __uint_as_float(((nmb << (31 - k)) & 0x80000000u) | cub);
           const float dn = fp[k] * sg;
-          const float hik = L.hi[pb][k][q], lok = L.lo[pb][k][q];
+          const float hik = hib[k], lok = lob[k];
           const bool take = act & (dn > hik);
           amb |= act & !take & !(dn < lok);
           mb ^= (uint32_t)take << k;
@@ -1293,8 +1321,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             const float sg = up ? cu : -cu;
             const float dn = sf[k] * sg;
             // idle lanes and padding rows have lo = hi = +inf: never taken
-            bool take = act & (dn > L.hi[pb][k][q]);
-            if (act && !take && !(dn < L.lo[pb][k][q])) {
+            bool take = act & (dn > L.hi[pb][q][k]);
+            if (act && !take && !(dn < L.lo[pb][q][k])) {
               const uint32_t pk = (uint32_t)(pos0 + (k ^ (int)xm));
               const uint32_t wd = pick_word(sweep_block(cid, sw, pk, p.k0, p.k1), pk & 3u);
               take = accept_exact(wd, beta, dn);
