@@ -741,6 +741,7 @@ __device__ __forceinline__ void swap2(float& a, float& b) {
 }
 __device__ __forceinline__ int fld(int q, int r) { return q * 32 + 4 * ((r >> 2) ^ (q & 7)) + (r & 3); }
 constexpr int kHL = 36;
+constexpr int kFG = 2;  // row groups per stage of the proposal wave's field sums
 template <int MW>
 struct __align__(16) PipeLds {
   float part[2][MW][64 * 32];  // [parity][matrix wave][fld(replica, row)]
@@ -777,19 +778,28 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
   const int read0 = blockIdx.x * p.rpw;
   const int h = lane >> 5;
-  // Two waves per SIMD: slot wl = w & 3 and tile = w >> 2.  Wave 0 runs the
-  // proposals; matrix wave (m = wl - 1, tile) computes the products of K
-  // range m for the 32 replicas of its tile; wave 4 only keeps the barriers
-  // (it shares wave 0's SIMD).
-  const int wl = w & 3, tile = w >> 2;
-  const bool mw = wl != 0;  // a matrix wave
-  const int m = wl - 1;
-  const int mi = mw ? tile * MW + m : 2 * MW;  // matrix-wave index 0..2MW-1
-  // Operand role (matrix waves): replica qt of this wave's tile.
-  const int qt = tile * 32 + (lane & 31);
-  const int gt = qt / R;
-  const bool actt = mw && (gt < p.rpw) && (read0 + gt < p.reads);
-  const size_t base_t = actt ? ((size_t)(read0 + gt) * R + (qt - gt * R)) * n : 0;
+  // Two waves per SIMD (SIMD w & 3).  Wave 0 runs the proposals; matrix
+  // wave w = 1..MW computes the products of K range m = w - 1 for both tiles
+  // of 32 replicas (each coupling fragment is loaded once for both); the
+  // helper waves MW+1.. stage the next blocks' J block, h and row sums,
+  // compute the acceptance brackets (those on SIMDs 1..MW) and, wave MW+1,
+  // the visit schedule.
+  static_assert(MW == 3, "eight waves: the proposal wave, three matrix waves, four helpers");
+  const bool mw = w >= 1 && w <= MW;  // a matrix wave
+  const bool hw = w > MW;             // a helper wave
+  const int m = mw ? w - 1 : 0;
+  const int hl = hw ? (w - MW - 1) * 64 + lane : -1;  // helper-lane index 0..64 (MW+1) - 1
+  // Operand role (matrix waves): replica qtt[tt] of tile tt.
+  int qtt[2], gtt[2];
+  bool actt[2];
+  size_t base_t[2];
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt) {
+    qtt[tt] = tt * 32 + (lane & 31);
+    gtt[tt] = qtt[tt] / R;
+    actt[tt] = mw && (gtt[tt] < p.rpw) && (read0 + gtt[tt] < p.reads);
+    base_t[tt] = actt[tt] ? ((size_t)(read0 + gtt[tt]) * R + (qtt[tt] - gtt[tt] * R)) * n : 0;
+  }
   // Replica role (every wave): lane = replica q of the group.
   const int q = lane;
   const int g = q / R, rr = q - g * R;
@@ -798,20 +808,22 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   const uint64_t cid = (uint64_t)read * R + rr;
   const int c0 = mw ? m * kb : 0;
   const int jn = mw ? max(0, min(kb, nblk - c0)) : 0;
-  const int mt = mw ? mi * 64 + lane : -1;  // matrix-lane index 0..128*MW-1
 
-  Bits<NW> sa;  // this wave's K range of its tile's spins, operand layout
-  sa.w = 0;
-  for (int j = 0; j < jn; ++j) {
-    const int c = c0 + j;
-    uint32_t va = 0;
+  Bits<NW> sa[2];  // this wave's K range of each tile's spins, operand layout
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      const int rel = I8 ? (16 * h + b) : ((b < 8) ? (8 * h + b) : (16 + 8 * h + (b - 8)));
-      const int pos = 32 * c + rel;
-      if (pos < n && actt && p.spins[base_t + pos] == 1) va |= 1u << b;
+  for (int tt = 0; tt < 2; ++tt) {
+    sa[tt].w = 0;
+    for (int j = 0; j < jn; ++j) {
+      const int c = c0 + j;
+      uint32_t va = 0;
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const int rel = I8 ? (16 * h + b) : ((b < 8) ? (8 * h + b) : (16 + 8 * h + (b - 8)));
+        const int pos = 32 * c + rel;
+        if (pos < n && actt[tt] && p.spins[base_t[tt] + pos] == 1) va |= 1u << b;
+      }
+      set_half<NW>(sa[tt], j, va);
     }
-    set_half<NW>(sa, j, va);
   }
   float E = 0.f, bestE = 0.f, gE = 0.f;
   int gsw = 0, gimp = 0;
@@ -836,7 +848,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   uint32_t vmask = 0x80008000u, vone = 0x3F803F80u;
   asm volatile("" : "+v"(vmask), "+v"(vone));
   constexpr int NST = I8 ? 1 : 2;
-  constexpr int NJB = (kB * kB + 128 * MW - 1) / (128 * MW);
+  constexpr int NHL = 64 * (MW + 1);  // helper lanes
+  constexpr int NJB = (kB * kB + NHL - 1) / NHL;
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
   Frag ring[PF][NST][NP];
   float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
@@ -866,12 +879,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
 #pragma unroll
       for (int pc = 0; pc < NP; ++pc) dst[st][pc] = base[(st * NP + pc) * 64 + lane];
   };
-  // Request row block c, visited with in-block mask xm: the first PF
-  // fragment stages of this wave's K range and (spread over the matrix
-  // lanes) J[blk][blk] in visit order, h and row sums.
-  auto prefetch_block = [&](int c, uint32_t xm) {
+  // Request row block c: the first PF fragment stages of this wave's K range
+  // (matrix waves).
+  auto prefetch_frags = [&](int c) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) load_stage(c, u, ring[u]);
+  };
+  // Request row block c, visited with in-block mask xm, spread over the
+  // helper lanes: J[blk][blk] in visit order, h and row sums.
+  auto prefetch_jb = [&](int c, uint32_t xm) {
     const int pos0 = 32 * c;
     // Every lane loads (an in-bounds address when its element is padding)
     // and publish_block zeroes the padding: no register is written here
@@ -879,7 +895,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     jb_ok = 0;
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = mt + e2 * 128 * MW;
+      const int e = hl + e2 * NHL;
       // slot t = e & 31 of visit row i holds visit column k = i + 1 + t;
       // visit index v is block row v ^ xm
       const int i = e >> 5, k = i + 1 + (e & 31);
@@ -888,21 +904,21 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       jb_ok |= (uint32_t)ok << e2;
       jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
     }
-    const bool hok = mt >= 0 && mt < kB && pos0 + mt < n;
+    const bool hok = hl >= 0 && hl < kB && pos0 + hl < n;
     jb_ok |= (uint32_t)hok << 31;
-    h_reg = p.h[hok ? pos0 + mt : 0];
-    if (!I8 && p.rowsum) rs_reg = p.rowsum[hok ? pos0 + mt : 0];
+    h_reg = p.h[hok ? pos0 + hl : 0];
+    if (!I8 && p.rowsum) rs_reg = p.rowsum[hok ? pos0 + hl : 0];
   };
   auto publish_block = [&](int pb) {
 #pragma unroll
     for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = mt + e2 * 128 * MW;
-      if (mt >= 0 && e < kB * kB) L.jb[pb][e] = ((jb_ok >> e2) & 1u) ? jb_reg[e2] : 0.f;
+      const int e = hl + e2 * NHL;
+      if (hl >= 0 && e < kB * kB) L.jb[pb][e] = ((jb_ok >> e2) & 1u) ? jb_reg[e2] : 0.f;
     }
-    if (mt >= 0 && mt < kB) {
+    if (hl >= 0 && hl < kB) {
       const bool hok = (jb_ok >> 31) & 1u;
-      L.hb[pb][mt] = hok ? h_reg : 0.f;
-      L.rsb[pb][mt] = (hok && !I8 && p.rowsum) ? rs_reg : 0.f;
+      L.hb[pb][hl] = hok ? h_reg : 0.f;
+      L.rsb[pb][hl] = (hok && !I8 && p.rowsum) ? rs_reg : 0.f;
     }
   };
   // Products of K step j (fragments fa) with this tile's spins hx.
@@ -948,15 +964,13 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       *reinterpret_cast<float4*>(&dst[fld(qcol, i)]) = x;
     }
   };
-  auto write_fields = [&](float* dst, const f32x16& acc, const i32x16& iacc) {
-    write_fields_col(dst, acc, iacc, qt);
-  };
+
   // Partial fields of row block cr over this wave's K range, skipping K block
   // ex (its product is added by the proposal wave once the block's spins are
   // final); ring holds the first PF stages of row block cr on entry.
   auto compute_part = [&](int cr, int ex, float* dst) {
-    f32x16 acc = {};
-    i32x16 iacc = {};
+    f32x16 acc[2] = {};
+    i32x16 iacc[2] = {};
     const int jx = ex - c0;  // the skipped stage, if it is in this wave's range
     constexpr int kUnrollJ = NW <= 8 ? (2 * NW + PF - 1) / PF : 1;
 #pragma unroll kUnrollJ
@@ -973,23 +987,25 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         // fragments: the load is issued after the products that read them);
         // slots past the 2 NW spin halves exist only when PF does not divide
         // 2 NW and are never live
-        if (j < 2 * NW && u < jrem && j != jx) mfma_step(ring[u], get_half<NW>(sa, j), acc, iacc);
+        if (j < 2 * NW && u < jrem && j != jx) {
+          mfma_step(ring[u], get_half<NW>(sa[0], j), acc[0], iacc[0]);
+          mfma_step(ring[u], get_half<NW>(sa[1], j), acc[1], iacc[1]);
+        }
         load_stage(cr, j + PF, ring[u]);
       }
     }
-    write_fields(dst, acc, iacc);
+    write_fields_col(dst, acc[0], iacc[0], qtt[0]);
+    write_fields_col(dst, acc[1], iacc[1], qtt[1]);
   };
   // Acceptance brackets of row block cr (in-block mask xm) for sweep counter
   // sw, stored by visit index, lane = replica (beta from L.rb): groups of 4
-  // rows (one Philox block each), one per matrix wave and the other two on
-  // wave 4, so that little of this work lands on the proposal wave's SIMD
-  // (the proposal chain is the critical path).
+  // rows (one Philox block each) dealt over the helpers on SIMDs 1..MW, so
+  // that none lands on the SIMD of the proposal chain.
   auto brackets = [&](int cr, uint32_t xm, uint64_t sw, int pb) {
     float cb, cz;
     bracket_factors(L.rb[q], cb, cz);
     const int pos0 = 32 * cr;
-    const int gstart = mw ? mi : 2 * MW, gstep = mw ? kB / 4 : 1;
-    for (int g4 = gstart; g4 < kB / 4; g4 += gstep) {
+    for (int g4 = w - MW - 2; g4 < kB / 4; g4 += MW) {
       const u32x4 blk = sweep_block(cid, sw, (uint32_t)(pos0 + 4 * g4), p.k0, p.k1);
       const uint32_t wd[4] = {blk.x, blk.y, blk.z, blk.w};
       float lo[4], hi[4];
@@ -1013,12 +1029,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       *reinterpret_cast<float4*>(&L.hi[pb][q][v4]) = make_float4(hi[0], hi[1], hi[2], hi[3]);
     }
   };
-  auto owner_of = [&](int c) { return 1 + c / kb; };  // wave owning K block c
-  auto publish_bits = [&](int c, int pb) {  // by the owners of block c (both tiles)
+  auto owner_of = [&](int c) { return 1 + c / kb; };  // matrix wave owning K block c
+  auto publish_bits = [&](int c, int pb) {  // by the owner of block c
     const int j = c - c0;
-    const uint32_t mine = get_half<NW>(sa, j);
-    const uint32_t got = (uint32_t)__shfl_xor((int)mine, 32, 64);  // the other half
-    if (lane < 32) L.bb[pb][qt] = block_of<I8>(mine, got);
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+      const uint32_t mine = get_half<NW>(sa[tt], j);
+      const uint32_t got = (uint32_t)__shfl_xor((int)mine, 32, 64);  // the other half
+      if (lane < 32) L.bb[pb][qtt[tt]] = block_of<I8>(mine, got);
+    }
   };
 
   // Visit schedule of sweep counter swn into parity par (wave 4, one lane
@@ -1097,13 +1116,17 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     const int cf = sched_block(0, 0), cs = sched_block(0, 1);
     const uint32_t mf = sched_mask(0, 0), ms = sched_mask(0, 1);
     if (mw) {
-      prefetch_block(cf, mf);
-      publish_block(0);
+      prefetch_frags(cf);
       compute_part(cf, -1, L.part[0][m]);
-      prefetch_block(cs, ms);
-      if (wl == owner_of(cf)) publish_bits(cf, 0);
+      prefetch_frags(cs);
+      if (w == owner_of(cf)) publish_bits(cf, 0);
     }
-    if (mw || w == 4) brackets(cf, mf, sw0, 0);
+    if (hw) {
+      prefetch_jb(cf, mf);
+      publish_block(0);
+      prefetch_jb(cs, ms);
+      if (w > MW + 1) brackets(cf, mf, sw0, 0);
+    }
   }
   __syncthreads();
 
@@ -1132,15 +1155,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         fa[st][pc] = fb[f * 64 + lane];
       }
   };
-  auto corr_products = [&](const FragSet& fa, uint32_t bits) {
+  auto corr_products = [&](const FragSet& fa, uint32_t bits, int ql) {  // ql: lane
 #pragma unroll
     for (int tl = 0; tl < 2; ++tl) {
       // column lane % 32 of tile tl is replica 32 tl + lane % 32
-      const uint32_t bt = (uint32_t)__shfl((int)bits, 32 * tl + (lane & 31), 64);
+      const uint32_t bt = (uint32_t)__shfl((int)bits, 32 * tl + (ql & 31), 64);
       f32x16 a = {};
       i32x16 ia = {};
       mfma_step(fa, half_of<I8>(bt, h), a, ia);
-      write_fields_col(L.corr, a, ia, 32 * tl + (lane & 31));
+      write_fields_col(L.corr, a, ia, 32 * tl + (ql & 31));
     }
   };
 
@@ -1163,6 +1186,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       uint32_t nacc = 0;
       for (int tb = 0; tb < nblk; ++tb, ++t) {
         const int pb = t & 1;
+        // The lane's LDS offsets are recomputed each block from an opaque
+        // copy of its index (a few VALU ops): hoisted out of the loop they
+        // would stay live across the chain, where registers are scarce.
+        int qo = q;
+        asm volatile("" : "+v"(qo));
         // block c of this step, its visit k being block row k ^ xm; c1 the next
         const int c = c_nx;
         const uint32_t xm = xm_nx;
@@ -1175,10 +1203,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         load_corr_frags(c1, c, fcorr);
         if (dry) {
           // ---- dry pass: f64 energy of the block's rows from its full fields
-          const uint32_t bits = L.bb[pb][q];
+          const uint32_t bits = L.bb[pb][qo];
           for (int i = 0; i < kB; ++i) {
             if (pos0 + i < n) {
-              const int o = fld(q, i);
+              const int o = fld(qo, i);
               double f = (double)L.part[pb][0][o];
               for (int mm = 1; mm < MW; ++mm) f += (double)L.part[pb][mm][o];
               f += (double)L.corr[o];
@@ -1187,9 +1215,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               eacc += si * (0.5 * f + (double)L.hb[pb][i]);
             }
           }
-          L.bnew[pb][q] = bits;
-          if (c1 == c) L.bb[pb ^ 1][q] = bits;  // the same block next (the owners skip it)
-          corr_products(fcorr, bits);
+          L.bnew[pb][qo] = bits;
+          if (c1 == c) L.bb[pb ^ 1][qo] = bits;  // the same block next (the owners skip it)
+          corr_products(fcorr, bits, qo);
           __syncthreads();  // B
           if (lastb) {
             if (act) {
@@ -1208,8 +1236,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         float hib[kB], lob[kB];
 #pragma unroll
         for (int t4 = 0; t4 < kB / 4; ++t4) {
-          const float4 a = *reinterpret_cast<const float4*>(&L.hi[pb][q][4 * t4]);
-          const float4 b = *reinterpret_cast<const float4*>(&L.lo[pb][q][4 * t4]);
+          const float4 a = *reinterpret_cast<const float4*>(&L.hi[pb][qo][4 * t4]);
+          const float4 b = *reinterpret_cast<const float4*>(&L.lo[pb][qo][4 * t4]);
           hib[4 * t4] = a.x, hib[4 * t4 + 1] = a.y, hib[4 * t4 + 2] = a.z, hib[4 * t4 + 3] = a.w;
           lob[4 * t4] = b.x, lob[4 * t4 + 1] = b.y, lob[4 * t4 + 2] = b.z, lob[4 * t4 + 3] = b.w;
         }
@@ -1220,28 +1248,42 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // in-group part is a compile-time permutation per uniform branch
           auto load_fields = [&](auto pc) __attribute__((always_inline)) {
             constexpr int P = decltype(pc)::value;
+            // kFG groups of 4 rows per stage: every LDS read of a stage is
+            // issued before its first sum, so a stage costs one LDS latency
 #pragma unroll
-            for (int r4 = 0; r4 < kB / 4; ++r4) {
-              const int gr = r4 ^ xg;
-              const int o = fld(q, 4 * gr);
-              float4 sm = *reinterpret_cast<const float4*>(&L.part[pb][0][o]);
+            for (int g0 = 0; g0 < kB / 4; g0 += kFG) {
+              float4 ld[kFG][MW + 3];
 #pragma unroll
-              for (int mm = 1; mm < MW; ++mm) {
-                const float4 a = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
-                sm.x = sm.x + a.x;
-                sm.y = sm.y + a.y;
-                sm.z = sm.z + a.z;
-                sm.w = sm.w + a.w;
+              for (int j = 0; j < kFG; ++j) {
+                const int gr = (g0 + j) ^ xg;
+                const int o = fld(qo, 4 * gr);
+#pragma unroll
+                for (int mm = 0; mm < MW; ++mm)
+                  ld[j][mm] = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
+                ld[j][MW] = *reinterpret_cast<const float4*>(&L.corr[o]);
+                ld[j][MW + 1] = *reinterpret_cast<const float4*>(&L.hb[pb][4 * gr]);
+                ld[j][MW + 2] = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
               }
-              const float4 cr = *reinterpret_cast<const float4*>(&L.corr[o]);
-              const float4 h4 = *reinterpret_cast<const float4*>(&L.hb[pb][4 * gr]);
-              const float4 s4 = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
-              const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y, sm.z + cr.z + s4.z,
-                                   sm.w + cr.w + s4.w};
-              const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
+              __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-              for (int e = 0; e < 4; ++e)
-                fp[4 * r4 + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
+              for (int j = 0; j < kFG; ++j) {
+                float4 sm = ld[j][0];
+#pragma unroll
+                for (int mm = 1; mm < MW; ++mm) {
+                  sm.x = sm.x + ld[j][mm].x;
+                  sm.y = sm.y + ld[j][mm].y;
+                  sm.z = sm.z + ld[j][mm].z;
+                  sm.w = sm.w + ld[j][mm].w;
+                }
+                const float4 cr = ld[j][MW], h4 = ld[j][MW + 1], s4 = ld[j][MW + 2];
+                const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y,
+                                     sm.z + cr.z + s4.z, sm.w + cr.w + s4.w};
+                const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                  fp[4 * (g0 + j) + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
+              }
+              __builtin_amdgcn_sched_barrier(0);
             }
           };
           switch (xm & 3u) {
@@ -1252,7 +1294,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           }
         }
         // spins in visit order (bit k = block row k ^ xm)
-        uint32_t mb = xor_permute_bits(L.bb[pb][q], xm);
+        uint32_t mb = xor_permute_bits(L.bb[pb][qo], xm);
         const uint32_t mb0 = mb;
         // bit k of mb is only changed at step k, so every step reads its
         // spin from the block's initial bits: sign bit = !spin, magnitude cu
@@ -1306,12 +1348,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // A decision fell inside its bracket: rerun the block, taking the
           // exact test for the decisions inside their brackets (rolled loop,
           // fields staged in LDS).
-          float* sf = &L.sfp[q * 33];
+          float* sf = &L.sfp[qo * 33];
           for (int r = 0; r < kB; ++r) {  // visit order, as fp
             const int br = r ^ (int)xm;
-            float v = L.part[pb][0][fld(q, br)];
-            for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(q, br)];
-            v = v + L.corr[fld(q, br)];
+            float v = L.part[pb][0][fld(qo, br)];
+            for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(qo, br)];
+            v = v + L.corr[fld(qo, br)];
             sf[r] = __builtin_fmaf(fsc, v + L.rsb[pb][br], L.hb[pb][br]);
           }
           mb = mb0;
@@ -1321,8 +1363,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             const float sg = up ? cu : -cu;
             const float dn = sf[k] * sg;
             // idle lanes and padding rows have lo = hi = +inf: never taken
-            bool take = act & (dn > L.hi[pb][q][k]);
-            if (act && !take && !(dn < L.lo[pb][q][k])) {
+            bool take = act & (dn > L.hi[pb][qo][k]);
+            if (act && !take && !(dn < L.lo[pb][qo][k])) {
               const uint32_t pk = (uint32_t)(pos0 + (k ^ (int)xm));
               const uint32_t wd = pick_word(sweep_block(cid, sw, pk, p.k0, p.k1), pk & 3u);
               take = accept_exact(wd, beta, dn);
@@ -1336,11 +1378,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         }
         {
           const uint32_t mbr = xor_permute_bits(mb, xm);  // back to block-row order
-          L.bnew[pb][q] = mbr;
+          L.bnew[pb][qo] = mbr;
           // the same block next (at a pass boundary): its spins are these
           // (the owners, whose copy is older, skip it)
-          if (c1 == c) L.bb[pb ^ 1][q] = mbr;
-          corr_products(fcorr, mbr);
+          if (c1 == c) L.bb[pb ^ 1][qo] = mbr;
+          corr_products(fcorr, mbr, qo);
         }
         dsum += (double)dpart;
         nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
@@ -1433,30 +1475,40 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         const uint32_t m2 = sched_mask(p2, i2);
         // ---- meanwhile: block c1's spins, fields except K block c, J / h and brackets
         if (mw) {
-          publish_block(pn);
           // c1's spins have not changed since its last visit unless c1 == c
           // (a pass boundary), which the proposal wave publishes itself
-          if (wl == owner_of(c1) && c1 != c) publish_bits(c1, pn);
+          if (w == owner_of(c1) && c1 != c) publish_bits(c1, pn);
           compute_part(c1, c, L.part[pn][m]);
-          prefetch_block(c2, m2);
+          prefetch_frags(c2);
         }
-        if ((mw || w == 4) && !last && !dry) brackets(c1, m1, sw, pn);
+        if (hw) {
+          publish_block(pn);
+          if (w > MW + 1 && !last && !dry) brackets(c1, m1, sw, pn);
+          prefetch_jb(c2, m2);
+        }
         __syncthreads();  // B
         // ---- the owners of block c take its new spins
-        if (mw && wl == owner_of(c)) set_half<NW>(sa, c - c0, half_of<I8>(L.bnew[pb][qt], h));
+        if (mw && w == owner_of(c)) {
+          set_half<NW>(sa[0], c - c0, half_of<I8>(L.bnew[pb][qtt[0]], h));
+          set_half<NW>(sa[1], c - c0, half_of<I8>(L.bnew[pb][qtt[1]], h));
+        }
         if (last) {
           if (!dry) {
             __syncthreads();
-            if (mw && p.best_sem == 0 && actt && L.flag[qt])
-              store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_rep + base_t);
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+              if (mw && p.best_sem == 0 && actt[tt] && L.flag[qtt[tt]])
+                store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype, p.best_rep + base_t[tt]);
             __syncthreads();
             // owners copy their part of a new global best
-            if (mw && p.best_sem == 1 && actt && L.src[gt] == qt - gt * R)
-              store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype,
-                                  p.best_state + (size_t)(read0 + gt) * n);
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt)
+              if (mw && p.best_sem == 1 && actt[tt] && L.src[gtt[tt]] == qtt[tt] - gtt[tt] * R)
+                store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype,
+                                    p.best_state + (size_t)(read0 + gtt[tt]) * n);
           }
           // the next pass's first block (after the exchanges: new betas)
-          if ((mw || w == 4) && !dry_next) brackets(c1, m1, sw_next, pn);
+          if (hw && w > MW + 1 && !dry_next) brackets(c1, m1, sw_next, pn);
           __syncthreads();
         }
         c = c1;
@@ -1467,7 +1519,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   }
 
   // ---- outputs
-  if (actt) store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.spins + base_t);
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt)
+    if (actt[tt]) store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype, p.spins + base_t[tt]);
   if (w == 0 && act) {
     if (p.accepted) atomicAdd(p.accepted + read, acc);
     p.energies[(size_t)read * R + rr] = E;
@@ -1481,8 +1535,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   if (p.best_sem == 1) {
     // never improved: best_state is still the view of states[0] (simulation.py:204)
     __syncthreads();
-    if (actt && qt - gt * R == 0 && !L.src[gt])
-      store_range<NW, I8>(sa, c0, kb, nblk, n, p.ptype, p.best_state + (size_t)(read0 + gt) * n);
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+      if (actt[tt] && qtt[tt] - gtt[tt] * R == 0 && !L.src[gtt[tt]])
+        store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype,
+                            p.best_state + (size_t)(read0 + gtt[tt]) * n);
   }
 }
 
