@@ -1299,6 +1299,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         // bit k of mb is only changed at step k, so every step reads its
         // spin from the block's initial bits: sign bit = !spin, magnitude cu
         const uint32_t nmb = ~mb0, cub = __float_as_uint(cu);
+        uint32_t vsgn = 0x80000000u;  // kept in a register (not a literal operand)
+        asm volatile("" : "+v"(vsgn));
         float dpart = 0.f;
         bool amb = false;
         // Rows of J[blk][blk] (the same for every lane) stream from LDS
@@ -1318,11 +1320,14 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         static_for<kB>([&](auto kc) __attribute__((always_inline)) {
           constexpr int k = decltype(kc)::value;
           // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
-          const float sg = __uint_as_float(((nmb << (31 - k)) & 0x80000000u) | cub);
+          const uint32_t shk = nmb << (31 - k);
+          const float sg = __uint_as_float((shk & vsgn) | (cub & ~vsgn));  // one bit insert
           const float dn = fp[k] * sg;
           const float hik = hib[k], lok = lob[k];
-          const bool take = act & (dn > hik);
-          amb |= act & !take & !(dn < lok);
+          // idle lanes and padding rows have lo = hi = +inf (and finite dn):
+          // never taken, never inside their bracket
+          const bool take = dn > hik;
+          amb |= !take & !(dn < lok);
           mb ^= (uint32_t)take << k;
           const float dS = take ? -sg : 0.f;
           // dS * fp[k] is exactly -dn or 0, so this rounds like dpart - dn
