@@ -1296,13 +1296,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         // spins in visit order (bit k = block row k ^ xm)
         uint32_t mb = xor_permute_bits(L.bb[pb][qo], xm);
         const uint32_t mb0 = mb;
+        // accepted flips, step k at bit 31 - k: one add with carry per step
+        uint32_t tk = 0;
         // bit k of mb is only changed at step k, so every step reads its
         // spin from the block's initial bits: sign bit = !spin, magnitude cu
         const uint32_t nmb = ~mb0, cub = __float_as_uint(cu);
         uint32_t vsgn = 0x80000000u;  // kept in a register (not a literal operand)
         asm volatile("" : "+v"(vsgn));
         float dpart = 0.f;
-        bool amb = false;
+        uint64_t amb = 0;  // lanes with a decision inside its bracket
         // Rows of J[blk][blk] (the same for every lane) stream from LDS
         // through a ring JR rows ahead of the chain.  Row k holds only the
         // columns right of the diagonal, shifted to the start of the row, so
@@ -1327,9 +1329,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // idle lanes and padding rows have lo = hi = +inf (and finite dn):
           // never taken, never inside their bracket
           const bool take = dn > hik;
-          amb |= !take & !(dn < lok);
-          mb ^= (uint32_t)take << k;
-          const float dS = take ? -sg : 0.f;
+          // the decision's lane mask, used as it is by the two selects below
+          const uint64_t tm = __builtin_amdgcn_ballot_w64(take);
+          amb |= __builtin_amdgcn_ballot_w64(!(dn < lok)) & ~tm;
+          float dS;  // take ? -sg : 0
+          asm("v_cndmask_b32_e64 %0, 0, -%1, %2" : "=v"(dS) : "v"(sg), "s"(tm));
+          {
+            uint64_t co;  // tk = 2 tk + take (the mask as carry-in)
+            asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(tk), "=s"(co) : "v"(tk), "s"(tm));
+            (void)co;
+          }
           // dS * fp[k] is exactly -dn or 0, so this rounds like dpart - dn
           dpart = __builtin_fmaf(dS, fp[k], dpart);
           static_for<(kB - 1 - k + 3) / 4>([&](auto tc) __attribute__((always_inline)) {
@@ -1349,7 +1358,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               jring[k % JR][t4] = jb4[(k + JR) * (kB / 4) + t4];
             });
         });
-        if (__builtin_expect(__any(amb), 0)) {
+        mb = mb0 ^ __builtin_bitreverse32(tk);
+        if (__builtin_expect(amb != 0, 0)) {
           // A decision fell inside its bracket: rerun the block, taking the
           // exact test for the decisions inside their brackets (rolled loop,
           // fields staged in LDS).
