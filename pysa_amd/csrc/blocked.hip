@@ -739,7 +739,13 @@ __device__ __forceinline__ void swap2(float& a, float& b) {
   a = b;
   b = t;
 }
-__device__ __forceinline__ int fld(int q, int r) { return q * 32 + 4 * ((r >> 2) ^ (q & 7)) + (r & 3); }
+// Field layout [replica][row]: the 16-byte groups of four rows are XOR
+// swizzled by the replica so that a 16-byte access by consecutive replicas
+// hits distinct banks in every aligned run of 8 lanes (32 banks) and of 16
+// lanes (64 banks: consecutive rows of 128 bytes alternate bank halves).
+__device__ __forceinline__ int fld(int q, int r) {
+  return q * 32 + 4 * ((r >> 2) ^ (q & 7) ^ ((q >> 3) & 1)) + (r & 3);
+}
 constexpr int kHL = 36;
 constexpr int kFG = 2;  // row groups per stage of the proposal wave's field sums
 template <int MW>
