@@ -906,7 +906,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       // visit index v is block row v ^ xm
       const int i = e >> 5, k = i + 1 + (e & 31);
       const int ri = i ^ (int)xm, rk = k ^ (int)xm;
-      const bool ok = e < kB * kB && k < kB && pos0 + ri < n && pos0 + rk < n;
+      const bool ok = e >= 0 && e < kB * kB && k < kB && pos0 + ri < n && pos0 + rk < n;
       jb_ok |= (uint32_t)ok << e2;
       jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
     }
