@@ -160,8 +160,9 @@ int pysa_amd_metropolis_device_noalloc(int64_t handle, int32_t reads, int32_t R,
  *                pysa/simulation.py:305-316), 1 = before it (Neal 2001)
  *   states       in/out chains*n spins
  *   log_w        out chains log-weights (f64, without log Z0)
- *   energies     out chains final energies as tracked (f64: recomputed every
- *                64 steps, updated by the accepted changes in between), or NULL
+ *   energies     out chains final energies as tracked (f64: recomputed with
+ *                the fields every 256 steps for float32 couplings, updated by
+ *                the accepted changes in between), or NULL
  */
 int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, int32_t reference,
                  double p, int32_t order, int32_t ordering, uint64_t seed,

@@ -158,7 +158,9 @@ int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
   p.use_pt = a.use_pt;
   p.logw = a.logw;
   p.best_sem = a.best_sem;
-  p.refresh = 32;
+  // f64 field refresh interval: float fields drift with the rounding of
+  // millions of row updates; double fields do not (refreshed only at start)
+  p.refresh = sizeof(T) == 4 ? 128 : 0;
   p.k0 = (uint32_t)a.seed;
   p.k1 = (uint32_t)(a.seed >> 32);
   p.sweep_offset = a.sweep_offset;
@@ -508,7 +510,7 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   hipError_t le;
   const double lp = reference == 1 ? log(p) : 0.0, l1p = reference == 1 ? log(1.0 - p) : 0.0;
   if (P->dtype == 0) {
-    pysa::AISParams<float> a = {(int)n, P->ptype, chains, nt, order, reference, ordering, 64,
+    pysa::AISParams<float> a = {(int)n, P->ptype, chains, nt, order, reference, ordering, 256,
                                 (uint32_t)seed, (uint32_t)(seed >> 32), lp, l1p,
                                 (const float*)P->dJt, (const float*)P->dJp, P->ldp,
                                 (const float*)P->dh, (const float*)dT,
@@ -516,7 +518,7 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
                                 (double*)dE, (double*)dW};
     le = pysa::launch_plain_ais<float>(a, st);
   } else {
-    pysa::AISParams<double> a = {(int)n, P->ptype, chains, nt, order, reference, ordering, 64,
+    pysa::AISParams<double> a = {(int)n, P->ptype, chains, nt, order, reference, ordering, 0,
                                  (uint32_t)seed, (uint32_t)(seed >> 32), lp, l1p,
                                  (const double*)P->dJt, (const double*)P->dJp, P->ldp,
                                  (const double*)P->dh, (const double*)dT,

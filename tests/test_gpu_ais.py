@@ -131,7 +131,7 @@ def test_ais_log_z_exact_rbm_14_10_float32(ordering):
 
 def test_ais_float32_matches_oracle_weights():
     """Float couplings: the kernel's weights (fields updated incrementally,
-    f64 refresh every 64 steps) against the oracle's (energy recomputed every
+    f64 refresh every 256 steps) against the oracle's (energy recomputed every
     step, as pysa/simulation.py:309) on the same Philox stream."""
     rng = np.random.default_rng(12)
     n, chains, nt = 40, 64, 300

@@ -99,7 +99,7 @@ def test_c3_wishart_full_n(strategy):
         rel = errs[form]
         # The tracked energy adds each sweep's accepted dE and rounds to f32
         # once per sweep (pysa/simulation.py:157-158).  The plain form resets
-        # it to the f64 energy with every field refresh (32 sweeps), so the
+        # it to the f64 energy with every field refresh (128 sweeps), so the
         # drift of its f32 fields does not accumulate.  The blocked form's
         # products and in-block updates use the same hi+lo bf16 couplings,
         # so its energy changes telescope to the energy of those couplings,
