@@ -208,6 +208,17 @@ def run_form(wl, form_name, order_name, steps, warmup, seed, errors=True):
     L = wl.lib
     form = FORM_CODES[form_name]
     order = L.ORDER[order_name]
+    shape = None
+    if form_name != "plain":
+        # the autotuner's launch shape for this batch (timed afresh, untimed
+        # here; results do not depend on it)
+        import tempfile
+
+        from pysa_amd import tune
+        path = os.path.join(tempfile.gettempdir(), f"pysa_amd_bench_tune_{os.getpid()}.json")
+        shape = tune.tune(wl.prob, wl.R, wl.reads, order, force=True, path=path)
+        log(f"[bench] {form_name}/{order_name}: tuned launch shape kernel {shape['kernel']}, "
+            f"tiles {shape['tiles']} ({shape['ms']})")
     for w in range(warmup):
         wl.step(form, order, seed + 7919 * (w + 1))
         log(f"[bench] {form_name}/{order_name}: warmup step {w + 1}/{warmup} done")
@@ -232,6 +243,10 @@ def run_form(wl, form_name, order_name, steps, warmup, seed, errors=True):
         "acceptance": acc / (wl.updates * steps),
         "t_read_s": float(np.mean(kms)) * 1e-3 / wl.reads,
     }
+    if shape is not None:
+        out["launch_shape"] = {"kernel": "pipelined" if shape["kernel"] == 1 else "split-K",
+                               "tiles_of_32_replicas": shape["tiles"], "block_size": 32,
+                               "tuning_ms": shape["ms"], "tuning_sweeps": shape["sweeps"]}
     if wl.e_gs is not None:
         gaps = (np.concatenate(bests) - wl.e_gs) / abs(wl.e_gs)
         out["best_gap_median"] = float(np.median(gaps))

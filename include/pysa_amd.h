@@ -86,8 +86,10 @@ int pysa_amd_problem_destroy(int64_t handle);
  *   seed             Philox key; sweep_offset shifts the sweep counter so a
  *                    resumed run continues the same stream
  *   form             PYSA_AMD_FORM_*; AUTO picks BLOCKED where it applies
- *                    (float32, sequential order, no replay, R <= 64, n <= 2048)
- *   block_b          reserved (blocked-form block size; 0 = default 32)
+ *                    (float32, sequential or block-random order, no replay,
+ *                    R <= 64, 32 <= n <= 2048)
+ *   block_b          blocked-form block size: 0 or 32 (the MFMA tile; other
+ *                    values give PYSA_AMD_ENOTSUP)
  *   replay_log_r     NULL, or reads*S*R*n log-uniforms indexed
  *                    [read][sweep][replica][position] (replay mode)
  *   replay_perm      NULL, or reads*S*R*n visit orders [read][sweep][replica][i]
@@ -191,6 +193,26 @@ int pysa_amd_energy(int64_t handle, const int8_t* states, int64_t count, double*
  * that the periodic f64 recomputation bounds.
  */
 int pysa_amd_plain_fields(int64_t handle, int64_t count, void* out);
+
+/*
+ * Launch shape of the MFMA form on this problem, for the autotuner
+ * (pysa_amd/tune.py).  No counterpart in the reference, whose numba loop has
+ * no launch shape.  Results do not depend on the shape, only the speed.
+ *   kernel  0 = by size (the pipelined kernel from three 32-spin blocks up),
+ *           1 = pipelined, 2 = split-K (the block-random order always runs
+ *           the pipelined kernel)
+ *   tiles   replicas per workgroup of the pipelined kernel in tiles of 32:
+ *           0 = by occupancy (one tile when two-tile workgroups would leave
+ *           compute units idle), 1 or 2 (R > 32 always takes 2)
+ */
+int pysa_amd_problem_set_shape(int64_t handle, int32_t kernel, int32_t tiles);
+
+/*
+ * The current device's compute units and architecture name (arch_len bytes,
+ * NUL-terminated), the device part of the autotuner's cache key.  No
+ * counterpart in the reference.
+ */
+int pysa_amd_device_info(int32_t* compute_units, char* arch, int32_t arch_len);
 
 /*
  * Many independent instances of different sizes in one launch.  PySA has no

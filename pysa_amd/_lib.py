@@ -44,6 +44,8 @@ _SIGNATURES = {
                      _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
     "pysa_amd_plain_fields": [_c_i64, _c_i64, _vp],
+    "pysa_amd_problem_set_shape": [_c_i64, _c_i32, _c_i32],
+    "pysa_amd_device_info": [ctypes.POINTER(_c_i32), ctypes.c_char_p, _c_i32],
     "pysa_amd_batched_device": [_c_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32,
                                 _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _vp, _c_u64, _c_u64,
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl),
@@ -79,6 +81,14 @@ def load(path=LIB_PATH):
         lib.pysa_amd_version.argtypes = []
         _lib = lib
         return lib
+
+
+def device_info():
+    """(compute units, architecture name) of the current device."""
+    cus = _c_i32(0)
+    buf = ctypes.create_string_buffer(64)
+    check(load().pysa_amd_device_info(ctypes.byref(cus), buf, 64))
+    return int(cus.value), buf.value.decode()
 
 
 def check(rc):
@@ -137,6 +147,12 @@ class Problem:
         out = np.empty(len(states), dtype=np.float64)
         check(load().pysa_amd_energy(self.handle, ptr(states), len(states), ptr(out)))
         return out
+
+    def set_shape(self, kernel=0, tiles=0):
+        """Launch shape of the MFMA form on this problem: kernel 0 = by size,
+        1 = pipelined, 2 = split-K; tiles of 32 replicas per workgroup 0 = by
+        occupancy, 1 or 2.  Results do not depend on it."""
+        check(load().pysa_amd_problem_set_shape(self.handle, int(kernel), int(tiles)))
 
     def plain_fields(self, count):
         """Local fields J s of the first `count` chains after the last

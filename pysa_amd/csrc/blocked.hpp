@@ -29,6 +29,11 @@ struct BlockedProblem {
   int rowsum_kind = -1;     // encoding the row sums were taken for
   float* jtl = nullptr;     // J^T as its bf16 piece sums (inexact encodings only)
   int jtl_kind = -1;
+  // Launch shape (results do not depend on it): kernel 0 = by size,
+  // 1 = pipelined, 2 = split-K; tiles of 32 replicas per workgroup of the
+  // pipelined kernel, 0 = by occupancy, 1 or 2.
+  int shape_kernel = 0;
+  int shape_tiles = 0;
 };
 
 void blocked_release(BlockedProblem& b);

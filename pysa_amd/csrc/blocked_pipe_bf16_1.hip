@@ -1,0 +1,12 @@
+// Pipelined MFMA kernel, 1 bf16 piece.
+#include "blocked_pipe.hpp"
+
+namespace pysa {
+namespace blk {
+
+hipError_t launch_pipe_bf16_1(const BParams& p, int nw, int nt, int groups, hipStream_t st) {
+  return launch_pipe_enc<1, false>(p, nw, nt, groups, st);
+}
+
+}  // namespace blk
+}  // namespace pysa

@@ -1,0 +1,12 @@
+// Pipelined MFMA kernel, int8 couplings.
+#include "blocked_pipe.hpp"
+
+namespace pysa {
+namespace blk {
+
+hipError_t launch_pipe_i8(const BParams& p, int nw, int nt, int groups, hipStream_t st) {
+  return launch_pipe_enc<1, true>(p, nw, nt, groups, st);
+}
+
+}  // namespace blk
+}  // namespace pysa

@@ -136,6 +136,8 @@ int check_run(const Problem* P, const RunArgs& a) {
     return fail(PYSA_AMD_EINVAL, "log_omega needs beta_sort and log_omegas");
   if (a.order < 0 || a.order > 3) return fail(PYSA_AMD_EINVAL, "order must be 0, 1, 2 or 3");
   if (a.best_sem < 0 || a.best_sem > 1) return fail(PYSA_AMD_EINVAL, "best_semantics must be 0 or 1");
+  if (a.block_b != 0 && a.block_b != 32)
+    return fail(PYSA_AMD_ENOTSUP, "block size must be 32 (the MFMA tile) or 0");
   if (a.form == PYSA_AMD_FORM_PLAIN && P->n > pysa::kPlainMaxN)
     return fail(PYSA_AMD_ENOTSUP, "n above the plain kernel's register budget (4096)");
   return 0;
@@ -569,6 +571,34 @@ int pysa_amd_memcpy_d2h(void* dst, const void* src, uint64_t bytes) {
 int pysa_amd_memcpy_d2d(void* dst, const void* src, uint64_t bytes) {
   g_err.clear();
   HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return 0;
+}
+
+int pysa_amd_device_info(int32_t* compute_units, char* arch, int32_t arch_len) {
+  g_err.clear();
+  if (!compute_units || (arch_len > 0 && !arch)) return fail(PYSA_AMD_EINVAL, "null argument");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  *compute_units = prop.multiProcessorCount;
+  if (arch_len > 0) {
+    strncpy(arch, prop.gcnArchName, (size_t)arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  return 0;
+}
+
+int pysa_amd_problem_set_shape(int64_t handle, int32_t kernel, int32_t tiles) {
+  g_err.clear();
+  Problem* P = get(handle);
+  if (!P) return fail(PYSA_AMD_EINVAL, "invalid problem handle");
+  if (kernel < 0 || kernel > 2)
+    return fail(PYSA_AMD_EINVAL, "kernel must be 0 (by size), 1 (pipelined) or 2 (split-K)");
+  if (tiles < 0 || tiles > 2) return fail(PYSA_AMD_EINVAL, "tiles must be 0 (by occupancy), 1 or 2");
+  std::lock_guard<std::mutex> lk(P->mu);
+  P->blocked.shape_kernel = kernel;
+  P->blocked.shape_tiles = tiles;
   return 0;
 }
 

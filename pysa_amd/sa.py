@@ -10,7 +10,9 @@ Engine options are keyword-only extras, all defaulted so that PySA calls are
 unchanged:
   form        'auto' (blocked MFMA form where eligible, else plain),
               'plain', 'blocked' or 'blocked_exact'
-  block_size  blocked-form block size B (0 = the tuned choice)
+  block_size  MFMA-form block size B: 32 (the MFMA tile) with the library's
+              launch shape, or 0 (default) for the launch shape the
+              autotuner picks (pysa_amd/tune.py, cached per batch shape)
   seed        Philox key (default: drawn from os.urandom, which leaves the
               global NumPy RNG untouched)
   sweep_offset  Philox sweep-counter offset (resume a stream)
@@ -30,6 +32,7 @@ import pandas as pd
 
 from . import _lib
 from . import ais as _ais
+from . import tune as _tune
 
 Vector = List[float]
 Matrix = List[List[float]]
@@ -218,6 +221,13 @@ class Solver(object):
             seed = secrets.randbits(64)
 
         prob = self._problem(ft)
+        if block_size not in (0, 32):
+            raise ValueError("block_size must be 0 (tuned launch shape) or 32 (the MFMA tile).")
+        if form_code in (_lib.FORM_BLOCKED, _lib.FORM_BLOCKED_EXACT):
+            if block_size == 0:
+                _tune.apply(prob, R, num_reads, order)
+            else:
+                prob.set_shape(0, 0)
         beta_idx = np.empty((num_reads, R), dtype=np.int64)
         best_state = np.empty((num_reads, N), dtype=np.int8)
         best_energy = np.empty(num_reads, dtype=ft)
