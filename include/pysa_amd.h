@@ -67,6 +67,17 @@ const char* pysa_amd_version(void);
 int pysa_amd_problem_create(int32_t n, int32_t problem_type, int32_t dtype, const void* J,
                             const void* h, int64_t* handle_out);
 
+/*
+ * The same from device memory (J and h already resident, e.g. torch tensors
+ * of a training loop): no host copy.  J need not be symmetric (its transpose
+ * is formed on the device).  The caller makes sure the work that wrote J and
+ * h has completed (synchronise its stream); the call returns when the
+ * problem's copies are made.  Serves the torch-native negative phase of RBM
+ * training (tutorials/rbm_pysa.py:111-140 rebuilds a Solver from host arrays).
+ */
+int pysa_amd_problem_create_device(int32_t n, int32_t problem_type, int32_t dtype,
+                                   const void* dJ, const void* dh, int64_t* handle_out);
+
 /* Release a problem and its device memory. */
 int pysa_amd_problem_destroy(int64_t handle);
 

@@ -45,6 +45,7 @@ _SIGNATURES = {
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
     "pysa_amd_plain_fields": [_c_i64, _c_i64, _vp],
     "pysa_amd_problem_set_shape": [_c_i64, _c_i32, _c_i32],
+    "pysa_amd_problem_create_device": [_c_i32, _c_i32, _c_i32, _vp, _vp, ctypes.POINTER(_c_i64)],
     "pysa_amd_device_info": [ctypes.POINTER(_c_i32), ctypes.c_char_p, _c_i32],
     "pysa_amd_batched_device": [_c_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32,
                                 _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _vp, _c_u64, _c_u64,

@@ -278,11 +278,92 @@ struct Staged {
 
 }  // namespace
 
+namespace {
+template <typename T>
+__global__ void transpose_kernel(const T* __restrict__ in, T* __restrict__ out, int n) {
+  __shared__ T tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  for (int k = ty; k < 32; k += 8)
+    if (by + k < n && bx + tx < n) tile[k][tx] = in[(size_t)(by + k) * n + bx + tx];
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8)
+    if (bx + k < n && by + tx < n) out[(size_t)(bx + k) * n + by + tx] = tile[tx][k];
+}
+
+// Device copies, transpose and padded rows of a problem whose J and h are
+// already in device memory (ordered after earlier work on the default stream).
+int create_from_device(Problem* P, const void* dJ, const void* dh) {
+  const int n = P->n;
+  const size_t es = esize(P->dtype), nn = (size_t)n * n;
+  hipError_t e = hipMalloc(&P->dJ, nn * es);
+  if (e == hipSuccess) e = hipMalloc(&P->dh, (size_t)n * es);
+  if (e == hipSuccess) e = hipMalloc(&P->dJt, nn * es);
+  if (e == hipSuccess) e = hipMemcpyAsync(P->dJ, dJ, nn * es, hipMemcpyDeviceToDevice, P->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(P->dh, dh, (size_t)n * es, hipMemcpyDeviceToDevice, P->stream);
+  if (e == hipSuccess) {
+    dim3 grid((n + 31) / 32, (n + 31) / 32);
+    if (P->dtype == 0)
+      hipLaunchKernelGGL(transpose_kernel<float>, grid, dim3(256), 0, P->stream,
+                         (const float*)P->dJ, (float*)P->dJt, n);
+    else
+      hipLaunchKernelGGL(transpose_kernel<double>, grid, dim3(256), 0, P->stream,
+                         (const double*)P->dJ, (double*)P->dJt, n);
+    e = hipGetLastError();
+  }
+  P->symmetric = false;
+  if (e == hipSuccess) {
+    P->ldp = ((n + 255) / 256) * 256;
+    if (P->ldp == n) {
+      P->dJp = P->dJt;
+    } else {
+      const size_t bytes = (size_t)n * P->ldp * es;
+      e = hipMalloc(&P->dJp, bytes);
+      if (e == hipSuccess) e = hipMemsetAsync(P->dJp, 0, bytes, P->stream);
+      if (e == hipSuccess)
+        e = hipMemcpy2DAsync(P->dJp, (size_t)P->ldp * es, P->dJt, (size_t)n * es, (size_t)n * es,
+                             n, hipMemcpyDeviceToDevice, P->stream);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+  if (e != hipSuccess) return fail(PYSA_AMD_EHIP, std::string("problem copy: ") + hipGetErrorString(e));
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 const char* pysa_amd_last_error(void) { return g_err.c_str(); }
 
 const char* pysa_amd_version(void) { return "pysa_amd 0.1.0 (gfx950)"; }
+
+
+int pysa_amd_problem_create_device(int32_t n, int32_t problem_type, int32_t dtype,
+                                   const void* dJ, const void* dh, int64_t* handle_out) {
+  g_err.clear();
+  if (!handle_out || !dJ || !dh) return fail(PYSA_AMD_EINVAL, "null argument");
+  if (n < 1) return fail(PYSA_AMD_EINVAL, "n must be >= 1");
+  if (problem_type != 0 && problem_type != 1)
+    return fail(PYSA_AMD_EINVAL, "problem_type must be 0 (ising) or 1 (qubo)");
+  if (dtype != 0 && dtype != 1) return fail(PYSA_AMD_EINVAL, "dtype must be 0 (f32) or 1 (f64)");
+  Problem* P = new Problem();
+  P->n = n;
+  P->ptype = problem_type;
+  P->dtype = dtype;
+  hipError_t e = hipGetDevice(&P->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete P;
+    return fail(PYSA_AMD_EHIP, std::string("problem stream: ") + hipGetErrorString(e));
+  }
+  const int rc = create_from_device(P, dJ, dh);
+  if (rc) {
+    delete P;
+    return rc;
+  }
+  *handle_out = reinterpret_cast<int64_t>(P);
+  return 0;
+}
 
 int pysa_amd_problem_create(int32_t n, int32_t problem_type, int32_t dtype, const void* J,
                             const void* h, int64_t* handle_out) {
