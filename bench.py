@@ -286,6 +286,38 @@ def tts_c3(forms, reads, sweeps, tmin, tmax, alpha, R=32, steps=2, seed=1003):
     return res
 
 
+def sparse_lattice(L=32, reads=512, R=32, sweeps=100, seed=1006):
+    """The sparse form against the dense plain form on a periodic L x L
+    lattice with +-1 couplings (4 nonzeros per row), random order, PT
+    (T 0.5 -> 3.0): same results bit for bit, device time of one pass each.
+    The sparse form reads 4 (column, value) pairs, 32 B, per accepted flip
+    against n x 4 B for the dense row, but runs half the waves (its update
+    needs more registers), and the per-visit work, not the row, bounds both."""
+    from pysa_amd import _lib
+
+    rng = np.random.default_rng(seed)
+    n = L * L
+    P = np.zeros((n, n), np.float32)
+    for i in range(L):
+        for j in range(L):
+            a = i * L + j
+            for b in (((i + 1) % L) * L + j, i * L + (j + 1) % L):
+                P[a, b] = P[b, a] = 2 * rng.integers(0, 2) - 1
+    wl = Workload(P, 0, reads, R, sweeps, 0.5, 3.0, seed)
+    res = {"workload": f"periodic {L}x{L} lattice Ising (n={n}), +-1 couplings, {reads} reads x "
+                       f"{R} replicas x {sweeps} sweeps, T 0.5->3.0, update_strategy='random'"}
+    states = {}
+    for name, form in (("plain", _lib.FORM_PLAIN), ("sparse", _lib.FORM_SPARSE)):
+        wl.step(form, _lib.ORDER["random"], seed)  # warm-up
+        ms = wl.step(form, _lib.ORDER["random"], seed + 1)
+        states[name] = wl.d_st.numpy()
+        res[name] = {"value": wl.updates / (ms * 1e-3), "ms": ms}
+        log(f"[bench] lattice {name}: {ms:.1f} ms device")
+    res["sparse_over_plain"] = res["sparse"]["value"] / res["plain"]["value"]
+    res["same_states"] = bool(np.array_equal(states["plain"], states["sparse"]))
+    return res
+
+
 def qubo_c2(forms, steps=2, warmup=1, seed=1002):
     """Dense QUBO N=512 (Q upper triangular N(0,1) off the diagonal and on
     it, passed as P = Q + Q^T - diag Q), 256 reads x 32 replicas x 2000
@@ -622,6 +654,7 @@ def main():
         line["blocked_over_plain"] = res["blocked"]["value"] / res["plain"]["value"]
     if not args.no_extra:
         line["qubo_c2"] = qubo_c2(forms)
+        line["sparse_lattice"] = sparse_lattice()
     if not args.no_tts:
         line["tts_c3"] = tts_c3(forms, args.tts_reads, args.tts_sweeps, args.tts_tmin,
                                 args.tts_tmax, args.tts_alpha)

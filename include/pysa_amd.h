@@ -51,6 +51,7 @@ extern "C" {
 #define PYSA_AMD_FORM_BLOCKED 1
 #define PYSA_AMD_FORM_AUTO 2
 #define PYSA_AMD_FORM_BLOCKED_EXACT 3
+#define PYSA_AMD_FORM_SPARSE 4
 
 /* Message describing the last failure on this thread ("" if none). */
 const char* pysa_amd_last_error(void);
@@ -98,7 +99,12 @@ int pysa_amd_problem_destroy(int64_t handle);
  *                    resumed run continues the same stream
  *   form             PYSA_AMD_FORM_*; AUTO picks BLOCKED where it applies
  *                    (float32, sequential or block-random order, no replay,
- *                    R <= 64, 32 <= n <= 2048)
+ *                    R <= 64, 32 <= n <= 2048), else PLAIN.  SPARSE is the
+ *                    plain form with an accepted flip updating only the
+ *                    nonzeros of its coupling row (same results bit for
+ *                    bit); it needs a symmetric J with at most min(n/4, 64)
+ *                    nonzeros per row, whose ELL copy pysa_amd_problem_create
+ *                    builds
  *   block_b          blocked-form block size: 0 or 32 (the MFMA tile; other
  *                    values give PYSA_AMD_ENOTSUP)
  *   replay_log_r     NULL, or reads*S*R*n log-uniforms indexed

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpysa_amd.so")
 
-FORM_PLAIN, FORM_BLOCKED, FORM_AUTO, FORM_BLOCKED_EXACT = 0, 1, 2, 3
+FORM_PLAIN, FORM_BLOCKED, FORM_AUTO, FORM_BLOCKED_EXACT, FORM_SPARSE = 0, 1, 2, 3, 4
 ORDER = {"sequential": 0, "random": 1, "random_shared": 2, "block_random": 3}
 
 _lib = None

@@ -56,6 +56,11 @@ struct Problem {
   void* dJp = nullptr;  // rows of dJt padded with zeros to ldp (== dJt when ldp == n)
   int ldp = 0;  // a multiple of 256
   void* dh = nullptr;
+  // sparse copy of a symmetric J with at most n/4 nonzeros per row: row i's
+  // nonzeros in ascending column order, padded to ell_d with column -1
+  int* d_ell_cols = nullptr;
+  void* d_ell_vals = nullptr;
+  int ell_d = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;  // calls on one problem are serialised
   // scratch reused across calls
@@ -68,6 +73,8 @@ struct Problem {
     if (dJt && dJt != dJ) (void)hipFree(dJt);
     if (dJ) (void)hipFree(dJ);
     if (dh) (void)hipFree(dh);
+    if (d_ell_cols) (void)hipFree(d_ell_cols);
+    if (d_ell_vals) (void)hipFree(d_ell_vals);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -145,6 +152,8 @@ int check_run(const Problem* P, const RunArgs& a) {
 
 template <typename T>
 int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
+  if (sizeof(T) == 8 && P->n > pysa::kPlainMaxN64)
+    return fail(PYSA_AMD_ENOTSUP, "float64 above n = 2048 is not supported by the plain kernel");
   const size_t n = P->n, cnt = (size_t)a.reads * a.R;
   HIP_TRY(P->fields.ensure(cnt * n * sizeof(T)));
   HIP_TRY(P->best_rep_e.ensure(cnt * sizeof(T)));
@@ -169,6 +178,10 @@ int run_plain(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
   p.Jt = (const T*)P->dJt;
   p.Jp = (const T*)P->dJp;
   p.ldp = P->ldp;
+  const bool sparse = a.form == PYSA_AMD_FORM_SPARSE;
+  p.ell_cols = sparse ? P->d_ell_cols : nullptr;
+  p.ell_vals = sparse ? (const T*)P->d_ell_vals : nullptr;
+  p.ell_d = sparse ? P->ell_d : 0;
   p.h = (const T*)P->dh;
   p.betas = (const T*)a.betas;
   p.bsort = a.bsort;
@@ -236,8 +249,14 @@ int run_device(Problem* P, const RunArgs& a, hipStream_t st, double* ms) {
     if (rc != 0) return fail(rc, why);
     return 0;
   }
-  if (form != PYSA_AMD_FORM_PLAIN) return fail(PYSA_AMD_EINVAL, "unknown form");
-  return run_plain<T>(P, a, st, ms);
+  if (form == PYSA_AMD_FORM_SPARSE && P->ell_d == 0)
+    return fail(PYSA_AMD_ENOTSUP,
+                "the sparse form needs a symmetric J with at most n/4 nonzeros per row");
+  if (form != PYSA_AMD_FORM_PLAIN && form != PYSA_AMD_FORM_SPARSE)
+    return fail(PYSA_AMD_EINVAL, "unknown form");
+  RunArgs b = a;
+  b.form = form;
+  return run_plain<T>(P, b, st, ms);
 }
 
 int metropolis_device(int64_t handle, const RunArgs& a, double* kernel_ms, void* stream) {
@@ -293,6 +312,39 @@ __global__ void transpose_kernel(const T* __restrict__ in, T* __restrict__ out, 
 
 // Device copies, transpose and padded rows of a problem whose J and h are
 // already in device memory (ordered after earlier work on the default stream).
+// The ELL copy of a symmetric host J when every row has at most n/4
+// nonzeros (else none: the sparse form is then unavailable).
+template <typename T>
+hipError_t build_ell(Problem* P, const T* J) {
+  const int n = P->n;
+  int d = 0;
+  for (int i = 0; i < n; ++i) {
+    int c = 0;
+    for (int j = 0; j < n; ++j) c += J[(size_t)i * n + j] != T(0);
+    d = std::max(d, c);
+  }
+  if (d == 0 || 4 * d > n || d > 64) return hipSuccess;
+  std::vector<int> cols((size_t)n * d, -1);
+  std::vector<T> vals((size_t)n * d, T(0));
+  for (int i = 0; i < n; ++i) {
+    int c = 0;
+    for (int j = 0; j < n; ++j)
+      if (J[(size_t)i * n + j] != T(0)) {
+        cols[(size_t)i * d + c] = j;
+        vals[(size_t)i * d + c] = J[(size_t)i * n + j];
+        ++c;
+      }
+  }
+  hipError_t e = hipMalloc(&P->d_ell_cols, cols.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&P->d_ell_vals, vals.size() * sizeof(T));
+  if (e == hipSuccess)
+    e = hipMemcpy(P->d_ell_cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(P->d_ell_vals, vals.data(), vals.size() * sizeof(T), hipMemcpyHostToDevice);
+  if (e == hipSuccess) P->ell_d = d;
+  return e;
+}
+
 int create_from_device(Problem* P, const void* dJ, const void* dh) {
   const int n = P->n;
   const size_t es = esize(P->dtype), nn = (size_t)n * n;
@@ -403,6 +455,8 @@ int pysa_amd_problem_create(int32_t n, int32_t problem_type, int32_t dtype, cons
     }
     P->symmetric = sym;
     if (sym) P->dJt = P->dJ;
+    if (sym)
+      e = dtype == 0 ? build_ell<float>(P, (const float*)J) : build_ell<double>(P, (const double*)J);
   }
   // Row-padded copy for the plain kernels' 16-byte row loads: whole
   // 256-element chunks per row.
@@ -571,6 +625,8 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   if (reference == 1 && !(p > 0.0 && p < 1.0)) return fail(PYSA_AMD_EINVAL, "Bernoulli p must be in (0, 1)");
   if (order < 0 || order > 3) return fail(PYSA_AMD_EINVAL, "order must be 0, 1, 2 or 3");
   if (P->n > pysa::kPlainMaxN) return fail(PYSA_AMD_ENOTSUP, "n above 4096");
+  if (P->dtype == 1 && P->n > pysa::kPlainMaxN64)
+    return fail(PYSA_AMD_ENOTSUP, "float64 above n = 2048 is not supported by the plain kernel");
   std::lock_guard<std::mutex> lk(P->mu);
   HIP_TRY(hipSetDevice(P->device));
   hipStream_t st = P->stream;

@@ -22,6 +22,11 @@ struct PTParams {
   const T* Jt;       // n*n, row p holds the coefficients J[i][p] (column p of J)
   const T* Jp;       // the rows of Jt padded with zeros to ldp (a multiple of 256)
   int ldp;
+  // Sparse form (null otherwise): the nonzeros of row p of the symmetric J
+  // in ascending column order, padded to ell_d entries per row with column -1
+  const int* ell_cols;  // n*ell_d
+  const T* ell_vals;    // n*ell_d
+  int ell_d;
   const T* h;        // n
   const T* betas;    // R, slot order (slot 0 = coldest)
   const int* bsort;  // R, slots sorted by ascending beta (log-omega)
@@ -72,6 +77,10 @@ hipError_t launch_plain_ais(const AISParams<T>& p, hipStream_t st);
 
 // Largest problem size the plain kernels hold in registers.
 constexpr int kPlainMaxN = 4096;
+// float64 chains above n = 2048 (64 register slots of doubles per lane) are
+// refused: that instantiation took a memory fault in testing whose cause is
+// not found, so its path stays off.
+constexpr int kPlainMaxN64 = 2048;
 
 // Many instances of different sizes in one launch (batched form).  Instance
 // k: n[k] spins, couplings scale[k] * S_k with S_k an int8 matrix (row-major,

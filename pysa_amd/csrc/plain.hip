@@ -59,6 +59,15 @@ __device__ __forceinline__ int elem_of(int lane, int g) {
 __device__ __forceinline__ int lane_of(int i) { return (i >> 2) & 63; }
 __device__ __forceinline__ int slot_of(int i) { return ((i >> 8) << 2) | (i & 3); }
 
+__device__ __forceinline__ uint32_t shfl_bits(uint32_t w, int src) {
+  return (uint32_t)__shfl((int)w, src, 64);
+}
+__device__ __forceinline__ uint64_t shfl_bits(uint64_t w, int src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)w, src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(w >> 32), src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 template <typename T>
 struct alignas(4 * sizeof(T)) T4 {
   T v[4];
@@ -169,6 +178,66 @@ struct Chain {
     return wave_sum(e);
   }
 
+  // The same from the nonzeros of a symmetric J (sparse form): row i's
+  // entries in ascending column order, so the f64 sums equal the dense
+  // ones term for term (a zero coupling adds exactly nothing).
+  __device__ double refresh_ell(const int* __restrict__ cols, const T* __restrict__ vals, int D,
+                                const T* h, int n, int ptype) {
+    const int lane = threadIdx.x & 63;
+    double e = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int i = elem_of(lane, g);
+      double a = 0.0;
+      for (int k = 0; k < D; ++k) {
+        const int j = (i < n) ? cols[(size_t)i * D + k] : -1;
+        // every lane takes part in the exchange (a disabled source lane reads as zero)
+        const B w = shfl_bits(bits, lane_of(j < 0 ? 0 : j));
+        if (j < 0) continue;
+        const bool upj = (w >> slot_of(j)) & 1;
+        if (ptype == 1 && !upj) continue;
+        a += (double)vals[(size_t)i * D + k] * (upj ? 1.0 : -1.0);
+      }
+      f[g] = (T)a;
+      if (i < n) {
+        const double si = ((bits >> g) & 1) ? 1.0 : (ptype == 0 ? -1.0 : 0.0);
+        e += si * (0.5 * a + (double)h[i]);
+      }
+    }
+    return wave_sum(e);
+  }
+
+  // Add m to f[g] for a wave-uniform slot g (a scalar branch tree on g's
+  // bits, so the register index stays a compile-time constant).
+  template <int LO, int HI>
+  __device__ __forceinline__ void add_at(int g, T m) {
+    if constexpr (HI - LO == 1) {
+      f[LO] += m;
+    } else {
+      constexpr int MID = (LO + HI) / 2;
+      if (g < MID)
+        add_at<LO, MID>(g, m);
+      else
+        add_at<MID, HI>(g, m);
+    }
+  }
+  // Add dS times row p of a symmetric sparse J (its ELL row: d <= 64
+  // entries, column -1 past the row's nonzeros): lane k loads entry k (one
+  // load latency for the whole row), then each nonzero's owner lane updates
+  // its register; the other lanes add zero, as the dense row would.
+  __device__ __forceinline__ void add_ell(const int* __restrict__ cols,
+                                          const T* __restrict__ vals, int d, T dS) {
+    const int lane = threadIdx.x & 63;
+    const int kl = lane < d ? lane : 0;
+    const int cl = cols[kl];
+    const T vl = vals[kl];
+    for (int k = 0; k < d; ++k) {
+      const int j = rl(cl, k);
+      if (j < 0) break;
+      add_at<0, G>(slot_of(j), lane == lane_of(j) ? rl(vl, k) * dS : T(0));
+    }
+  }
+
   __device__ int count_up(int n) const {
     const int lane = threadIdx.x & 63;
     int c = 0;
@@ -248,13 +317,19 @@ struct SweepCtx {
   uint32_t k0, k1;
   uint64_t sw;
   double lratio;
+  // sparse form: ELL rows of J (ell_d > 0), else the dense padded rows
+  const int* ell_cols = nullptr;
+  const void* ell_vals = nullptr;
+  int ell_d = 0;
 };
 
 // One sweep of one chain (simulation.py:41-76): visit order from the
 // sequential order, a Feistel permutation (per chain, or shared by all chains
 // of the sweep) or a replayed order; an accepted flip loads its coupling row
 // from global memory (L2 / Infinity Cache resident) as 16-byte vectors.
-template <typename T, int G, bool AIS>
+// SP: the sparse form (an accepted flip updates the nonzeros of its ELL row),
+// a separate instantiation so that the dense kernels keep their registers.
+template <typename T, int G, bool AIS, bool SP = false>
 __device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const SweepCtx& c,
                                const T* __restrict__ Jp, int ldp, T beta, uint64_t cid,
                                const double* rep_lr, const int* rep_perm,
@@ -302,16 +377,27 @@ __device__ double sweep_direct(Chain<T, G>& ch, bool active, const T* hs, const 
         ++acc;
         od += up ? -1 : 1;
         if (lane == pl) ch.bits ^= (typename Chain<T, G>::B(1) << ps);
-        ch.add_row_padded(Jp + (size_t)pos * ldp, ldp,
-                          (c.ptype == 0) ? (up ? T(-2) : T(2)) : (up ? T(-1) : T(1)));
+        const T dS = (c.ptype == 0) ? (up ? T(-2) : T(2)) : (up ? T(-1) : T(1));
+        if constexpr (SP)
+          ch.add_ell(c.ell_cols + (size_t)pos * c.ell_d,
+                     static_cast<const T*>(c.ell_vals) + (size_t)pos * c.ell_d, c.ell_d, dS);
+        else
+          ch.add_row_padded(Jp + (size_t)pos * ldp, ldp, dS);
       }
     }
   }
   return sum;
 }
 
-template <typename T, int G>
-__global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(PTParams<T> p) {
+// Waves per workgroup of the PT kernel: the sparse form's update and refresh
+// need more registers per lane, so it runs half as many waves.
+template <typename T, int G, bool SP>
+constexpr int pt_waves() {
+  return SP ? (kernel_waves<T, G>() > 4 ? kernel_waves<T, G>() / 2 : 4) : kernel_waves<T, G>();
+}
+
+template <typename T, int G, bool SP>
+__global__ __launch_bounds__((64 * pt_waves<T, G, SP>())) void plain_pt_kernel(PTParams<T> p) {
   const int read = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
   const int R = p.R, n = p.n;
@@ -346,6 +432,9 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(P
   c.k0 = p.k0;
   c.k1 = p.k1;
   c.lratio = 0.0;
+  c.ell_cols = p.ell_cols;
+  c.ell_vals = p.ell_vals;
+  c.ell_d = p.ell_d;
   unsigned long long acc = 0;
   for (int s = 0; s < p.S; ++s) {
     c.sw = (uint64_t)s + p.sweep_offset;
@@ -363,7 +452,11 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(P
         Chain<T, G> ch;
         const size_t off = ((size_t)read * R + r) * n;
         ch.load(p.fields + off, p.spins + off, n);
-        const double ex = ch.refresh(p.Jt, s_h, n, p.ptype);
+        double ex;
+        if constexpr (SP)
+          ex = ch.refresh_ell(p.ell_cols, p.ell_vals, p.ell_d, s_h, n, p.ptype);
+        else
+          ex = ch.refresh(p.Jt, s_h, n, p.ptype);
         ch.store(p.fields + off, p.spins + off, n, p.ptype);
         if (lane == 0) {
           if (s == 0)
@@ -392,7 +485,7 @@ __global__ __launch_bounds__((64 * kernel_waves<T, G>())) void plain_pt_kernel(P
       ch.load(p.fields + off, p.spins + off, n);
       const size_t rb = (((size_t)read * p.S + s) * R + r) * (size_t)n;
       int od;
-      const double d = sweep_direct<T, G, false>(ch, true, s_h, c, p.Jp, p.ldp, s_bs[r],
+      const double d = sweep_direct<T, G, false, SP>(ch, true, s_h, c, p.Jp, p.ldp, s_bs[r],
                                                  (uint64_t)read * R + r,
                                                  p.rep_lr ? p.rep_lr + rb : nullptr,
                                                  p.rep_perm ? p.rep_perm + rb : nullptr, acc, od);
@@ -584,8 +677,13 @@ int regs_for(int n) {
 
 template <typename T, int G>
 hipError_t launch_pt_shape(const PTParams<T>& p, hipStream_t st) {
-  const int W = min(p.R, kernel_waves<T, G>());
-  hipLaunchKernelGGL((plain_pt_kernel<T, G>), dim3(p.reads), dim3(64 * W), 0, st, p);
+  if (p.ell_d > 0) {
+    const int W = min(p.R, pt_waves<T, G, true>());
+    hipLaunchKernelGGL((plain_pt_kernel<T, G, true>), dim3(p.reads), dim3(64 * W), 0, st, p);
+  } else {
+    const int W = min(p.R, pt_waves<T, G, false>());
+    hipLaunchKernelGGL((plain_pt_kernel<T, G, false>), dim3(p.reads), dim3(64 * W), 0, st, p);
+  }
   return hipGetLastError();
 }
 

@@ -9,7 +9,9 @@ best tracking and log-omega of all reads run in one call into the HIP engine
 Engine options are keyword-only extras, all defaulted so that PySA calls are
 unchanged:
   form        'auto' (blocked MFMA form where eligible, else plain),
-              'plain', 'blocked' or 'blocked_exact'
+              'plain', 'sparse' (the plain form updating only the nonzeros
+              of a row: symmetric J, at most min(n/4, 64) nonzeros per row),
+              'blocked' or 'blocked_exact'
   block_size  MFMA-form block size B: 32 (the MFMA tile) with the library's
               launch shape, or 0 (default) for the launch shape the
               autotuner picks (pysa_amd/tune.py, cached per batch shape)
@@ -254,8 +256,8 @@ class Solver(object):
         if engine_stats is not None:
             upd = float(N) * num_sweeps * R * num_reads
             engine_stats.update({
-                'form': {_lib.FORM_BLOCKED: 'blocked', _lib.FORM_BLOCKED_EXACT: 'blocked_exact'}.get(
-                    form_code, 'plain'),
+                'form': {_lib.FORM_BLOCKED: 'blocked', _lib.FORM_BLOCKED_EXACT: 'blocked_exact',
+                         _lib.FORM_SPARSE: 'sparse'}.get(form_code, 'plain'),
                 'order': order, 'seed': int(seed), 'kernel_ms': kms.value, 'wall_s': wall,
                 'accepted': accepted.copy(), 'updates': upd,
                 'updates_per_s': upd / (kms.value * 1e-3) if kms.value > 0 else float('nan'),
@@ -372,6 +374,17 @@ class Solver(object):
                 raise ValueError("Cannot initialize system.")
         return _init
 
+    def _sparse(self):
+        """Whether the couplings qualify for the sparse form (symmetric, at
+        most min(n/4, 64) nonzeros in every row)."""
+        if getattr(self, '_is_sparse', None) is None:
+            J = self.couplings
+            n = self.n_vars
+            d = int(np.count_nonzero(J, axis=1).max()) if n > 0 else 0
+            self._is_sparse = bool(n > 0 and np.array_equal(J, J.T) and 0 < d <= 64 and
+                                   4 * d <= n)
+        return self._is_sparse
+
     def _resolve_form(self, form, update_strategy, ft, R, replay):
         """Engine form and visit order.
 
@@ -384,7 +397,7 @@ class Solver(object):
         random order of the 32-spin blocks per sweep with a random XOR
         shuffle inside each block, shared by all replicas; n > 64).  'auto'
         takes the MFMA form when it applies."""
-        if form not in ('auto', 'plain', 'blocked', 'blocked_exact'):
+        if form not in ('auto', 'plain', 'sparse', 'blocked', 'blocked_exact'):
             raise ValueError(f"form='{form}' not recognized.")
         blocked_order = 'sequential' if update_strategy == 'sequential' else 'block_random'
         ok = (ft == np.float32 and _BLOCKED_MIN_N <= self.n_vars <= _BLOCKED_MAX_N and
@@ -392,6 +405,11 @@ class Solver(object):
               (blocked_order == 'sequential' or self.n_vars > _BLOCKED_RANDOM_MIN_N))
         if form == 'auto':
             form = 'blocked' if ok else 'plain'
+        if form == 'sparse':
+            if not self._sparse():
+                raise ValueError("form='sparse' needs a symmetric problem with at most "
+                                 "min(n/4, 64) nonzero couplings per row.")
+            return _lib.FORM_SPARSE, _lib.ORDER[update_strategy]
         if form in ('blocked', 'blocked_exact'):
             if not ok:
                 raise ValueError(
