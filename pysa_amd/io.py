@@ -11,7 +11,8 @@ formats its tutorials read.
   passes to ``Solver``.
 * ``load_gs_energy`` -- the ``name<TAB>energy`` lines of
   tutorials/gs_energies.txt.
-* ``to_csr`` -- the sparse form the engine's sparse kernel takes.
+* ``to_csr`` -- CSR arrays of a problem matrix (the engine's sparse form
+  builds its own ELL copy from the dense matrix it is given).
 """
 
 import warnings
