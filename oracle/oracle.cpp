@@ -126,8 +126,12 @@ struct Perm {
   }
 };
 
+// chain bits 24-39 and sweep bits 32-47 share the last counter word
+uint32_t chain_sweep_hi(uint64_t chain, uint64_t sweep) {
+  return (uint32_t)((chain >> 24) & 0xFFFFu) | ((uint32_t)((sweep >> 32) & 0xFFFFu) << 16);
+}
 W4 ctr_sweep(uint64_t chain, uint64_t sweep, uint32_t pos) {
-  W4 c = {{pos >> 2, (uint32_t)sweep, ((uint32_t)chain & 0xFFFFFFu), (uint32_t)(chain >> 24)}};
+  W4 c = {{pos >> 2, (uint32_t)sweep, ((uint32_t)chain & 0xFFFFFFu), chain_sweep_hi(chain, sweep)}};
   return c;
 }
 W4 ctr_pt(uint64_t read, uint64_t sweep, uint32_t pair) {
@@ -137,7 +141,7 @@ W4 ctr_pt(uint64_t read, uint64_t sweep, uint32_t pair) {
 }
 W4 ctr_perm(uint64_t chain, uint64_t sweep, uint32_t purpose) {
   W4 c = {{0u, (uint32_t)sweep, ((uint32_t)chain & 0xFFFFFFu) | (purpose << 24),
-           (uint32_t)(chain >> 24)}};
+           chain_sweep_hi(chain, sweep)}};
   return c;
 }
 
@@ -160,7 +164,7 @@ struct BlockOrder {
     out.reserve(n);
     for (int t = 0; t < nblk; ++t) {
       const uint32_t b = blocks.at((uint32_t)t);
-      W4 c = {{b >> 2, (uint32_t)sweep, 5u << 24, 0u}};
+      W4 c = {{b >> 2, (uint32_t)sweep, 5u << 24, (uint32_t)((sweep >> 32) & 0xFFFFu)}};
       const uint32_t m = philox10(c, k0, k1).v[b & 3] & 31u;
       for (uint32_t k = 0; k < 32; ++k) {
         const int pos = (int)(32 * b + (k ^ m));

@@ -187,16 +187,21 @@ PYSA_HD uint32_t feistel_perm(const FeistelKey& f, uint32_t i) {
   return y;
 }
 
-// Counter layouts (key = seed words):
-//   sweep uniforms:  (pos >> 2, sweep, chain_lo | purpose<<24, chain_hi)
-//   PT uniforms:     (pair >> 2, sweep, purpose<<24, read)
-//   Feistel keys:    (0, sweep, chain_lo | purpose<<24, chain_hi)
+// Counter layouts (key = seed words); chain < 2^40 and sweep < 2^48, with
+// chain_lo = chain bits 0-23, chain_hi = bits 24-39, sweep_lo / sweep_hi =
+// bits 0-31 / 32-47:
+//   sweep uniforms:  (pos >> 2, sweep_lo, chain_lo | purpose<<24, chain_hi | sweep_hi<<16)
+//   PT uniforms:     (pair >> 2, sweep_lo, purpose<<24 | sweep_hi, read)
+//   Feistel keys:    (0, sweep_lo, chain_lo | purpose<<24, chain_hi | sweep_hi<<16)
+PYSA_HD uint32_t chain_sweep_hi(uint64_t chain, uint64_t sweep) {
+  return (uint32_t)((chain >> 24) & 0xFFFFu) | ((uint32_t)((sweep >> 32) & 0xFFFFu) << 16);
+}
 PYSA_HD u32x4 sweep_block(uint64_t chain, uint64_t sweep, uint32_t pos, uint32_t k0, uint32_t k1) {
   u32x4 c;
   c.x = pos >> 2;
   c.y = (uint32_t)sweep;
   c.z = ((uint32_t)chain & 0x00FFFFFFu) | (kStreamSweep << 24);
-  c.w = (uint32_t)(chain >> 24);
+  c.w = chain_sweep_hi(chain, sweep);
   return philox4x32_10(c, k0, k1);
 }
 
@@ -214,7 +219,7 @@ PYSA_HD u32x4 perm_keys(uint64_t chain, uint64_t sweep, uint32_t purpose, uint32
   c.x = 0;
   c.y = (uint32_t)sweep;
   c.z = ((uint32_t)chain & 0x00FFFFFFu) | (purpose << 24);
-  c.w = (uint32_t)(chain >> 24);
+  c.w = chain_sweep_hi(chain, sweep);
   return philox4x32_10(c, k0, k1);
 }
 
@@ -226,7 +231,7 @@ PYSA_HD u32x4 perm_keys(uint64_t chain, uint64_t sweep, uint32_t purpose, uint32
 // so each single-spin Metropolis update keeps the Boltzmann distribution
 // stationary, as with any random scan.
 //   block keys:  perm_keys(0, sweep, kStreamBlock)
-//   masks:       word b & 3 of Philox((b >> 2, sweep, kStreamMask << 24, 0)), low 5 bits
+//   masks:       word b & 3 of Philox((b >> 2, sweep_lo, kStreamMask << 24, sweep_hi)), low 5 bits
 PYSA_HD FeistelKey block_order_key(uint32_t nblk, uint64_t sweep, uint32_t k0, uint32_t k1) {
   return feistel_make(nblk, perm_keys(0ull, sweep, kStreamBlock, k0, k1));
 }
@@ -235,7 +240,7 @@ PYSA_HD uint32_t block_mask(uint64_t sweep, uint32_t b, uint32_t k0, uint32_t k1
   c.x = b >> 2;
   c.y = (uint32_t)sweep;
   c.z = kStreamMask << 24;
-  c.w = 0u;
+  c.w = (uint32_t)((sweep >> 32) & 0xFFFFu);
   return pick_word(philox4x32_10(c, k0, k1), b & 3u) & 31u;
 }
 // Position visited at step i (0 <= i < 32 * nblk) of the sweep, or -1 for a

@@ -198,3 +198,26 @@ def _check_recompute_offset(order, ptype):
     assert_same(got, ref)
     e64 = oracle.energy_f64(J, h, got["states"].reshape(-1, n)).reshape(reads, R)
     assert np.array_equal(got["energies"].astype(np.float64), e64 + 7.0)
+
+
+@pytest.mark.parametrize("form,order", [(_lib.FORM_PLAIN, 1), (_lib.FORM_BLOCKED, 3)])
+def test_sweep_counter_past_32_bits(form, order):
+    """sweep_offset + sweep past 2^32 keeps a distinct stream (the counter
+    carries 48 sweep bits): equal to the oracle, and different from the run
+    whose offset is 2^32 higher (a 32-bit counter would repeat it)."""
+    rng = np.random.default_rng(61 + order)
+    n, R, reads, S = 96, 8, 3, 4
+    J, h = int_problem(n, rng, -3, 3, 1)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    betas = (1 / np.geomspace(0.8, 4.0, R)).astype(np.float32)
+    st = random_spins((reads, R, n), rng, 0)
+    en = energies_of(J, h, st, 0, np.float32)
+    off = (1 << 32) - 2
+    ref = oracle.metropolis(J, h, 0, betas, st, en, S, order=order, seed=5, sweep_offset=off,
+                            nthreads=8)
+    got = engine_metropolis(J, h, 0, betas, st, en, S, order=order, seed=5, sweep_offset=off,
+                            form=form)
+    assert_same(got, ref)
+    low = engine_metropolis(J, h, 0, betas, st, en, S, order=order, seed=5,
+                            sweep_offset=off - (1 << 32) + (1 << 33), form=form)
+    assert not np.array_equal(got["states"], low["states"])
