@@ -342,10 +342,13 @@ def qubo_c2(forms, steps=2, warmup=1, seed=1002):
 def batched_c4(sweeps=2000, count=2048, loop_instances=64, seed=1004):
     """Sherrington-Kirkpatrick batch: `count` instances, n uniform in
     [32, 768], J = +-1/sqrt(n) (int8 signs + scale), h = 0, 4 reads x 8
-    replicas each, T 0.3 -> 1.5, random order.  One batched launch with inputs
-    resident in HBM, against a per-instance loop (one plain-form engine call
-    per instance, as a loop over Solver objects would make), timed by device
-    time (HIP events) over the first `loop_instances` instances."""
+    replicas each, T 0.3 -> 1.5, random order.  One batched call with inputs
+    resident in HBM, in the MFMA form (block-random order: one workgroup per
+    instance, int8 block products; instances below 65 spins on the plain
+    kernel in the same call) and in the plain form (a fresh permutation per
+    replica and sweep), against a per-instance loop (one plain-form engine
+    call per instance, as a loop over Solver objects would make), timed by
+    device time (HIP events) over the first `loop_instances` instances."""
     from pysa_amd import _lib, batched, instances
 
     reads, R = 4, 8
@@ -369,26 +372,44 @@ def batched_c4(sweeps=2000, count=2048, loop_instances=64, seed=1004):
     d_acc = D(count * reads, np.uint64)
     L = _lib
 
-    def launch(sd):
+    def launch(sd, form, order):
         d_st.copy_from(d_st0)
         d_en.copy_from(d_en0)
         kms = L._c_dbl(0.0)
         L.check(L.load().pysa_amd_batched_device(
             count, dev["n"].ptr, dev["ld"].ptr, dev["j_off"].ptr, dev["signs"].ptr,
-            dev["scale"].ptr, None, None, 0, reads, R, sweeps, 1, 1, 0, d_betas.ptr, sd, 0,
+            dev["scale"].ptr, None, None, 0, reads, R, sweeps, order, 1, 0, d_betas.ptr, sd, 0,
             d_soff.ptr, d_st.ptr, d_en.ptr, d_bi.ptr, d_bs.ptr, d_be.ptr, d_bw.ptr, d_acc.ptr,
-            L.ctypes.byref(kms), None))
+            form, L.ctypes.byref(kms), None))
         return kms.value
 
-    launch(1)
-    log("[bench] batched C4: warmup launch done")
-    kms = launch(2)
     updates = float(total) * sweeps
-    accs = d_acc.numpy().astype(np.float64)
-    acc = float(accs.sum())
-    # one n_k-byte int8 coupling row per accepted flip of instance k
-    row_bytes = float(np.sum(accs * np.repeat(ns, reads)))
-    log(f"[bench] batched C4: {kms:.1f} ms device")
+    out = {}
+    for name, form, order in (("blocked", L.FORM_BLOCKED, 3), ("plain", L.FORM_PLAIN, 1)):
+        launch(1, form, order)
+        log(f"[bench] batched C4 {name}: warmup launch done")
+        kms = launch(2, form, order)
+        accs = d_acc.numpy().astype(np.float64)
+        r = {"order": {1: "random", 3: "block_random"}[order], "kernel_ms": kms,
+             "value": updates / (kms * 1e-3), "acceptance": float(accs.sum()) / updates}
+        if name == "plain":
+            # one n_k-byte int8 coupling row per accepted flip of instance k,
+            # L2-resident (every matrix is at most 768 x 768 bytes)
+            gbs = float(np.sum(accs * np.repeat(ns, reads))) / (kms * 1e-3) / 1e9
+            r["roofline"] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                             "note": "int8 coupling row per accepted flip; served from L2"}
+        else:
+            # int8 block products: 2 n ops per spin update of the instances
+            # the MFMA form takes (n >= 65)
+            mf = ns >= 65
+            tops = float(np.sum(2.0 * ns[mf] * ns[mf] * reads * R * sweeps)) / (kms * 1e-3) / 1e12
+            r["roofline"] = {"bound": "mfma", "achieved": tops, "peak": 5000.0,
+                             "unit": "TOP/s", "frac": tops / 5000.0, "traffic": None,
+                             "note": "int8 MFMA, 2n ops per spin update, instances n >= 65 "
+                                     f"({int(mf.sum())} of {count})"}
+        out[name] = r
+        log(f"[bench] batched C4 {name}: {kms:.1f} ms device")
     # Per-instance loop: one plain-form engine call per instance (the batch
     # as a caller's loop would run it), first `loop_instances` instances,
     # device time of each call.
@@ -419,26 +440,24 @@ def batched_c4(sweeps=2000, count=2048, loop_instances=64, seed=1004):
     for a in list(dev.values()) + [d_betas, d_soff, d_st0, d_en0, d_st, d_en, d_bi, d_bs, d_be,
                                    d_bw, d_acc]:
         a.free()
-    row_gbs = row_bytes / (kms * 1e-3) / 1e9
+    loop_rate = loop_upd / (loop_ms * 1e-3)
+    best = out["blocked"]
     return {
         "workload": f"{count} SK instances, n uniform in [32, 768] (sum {int(ns.sum())}), "
                     f"J = +-1/sqrt(n) int8 signs x scale, h = 0, {reads} reads x {R} replicas x "
                     f"{sweeps} sweeps, T 0.3->1.5, random order, PT",
         "updates": updates,
-        "kernel_ms": kms,
-        "value": updates / (kms * 1e-3),
+        "kernel_ms": best["kernel_ms"],
+        "value": best["value"],
         "unit": "spin_updates/s",
-        "acceptance": acc / updates,
-        # algorithmic row stream of the batched kernel (rows are L2-resident:
-        # every instance's int8 matrix is at most 768 x 768 bytes)
-        "roofline": {"bound": "hbm", "achieved": row_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": row_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "note": "int8 coupling row per accepted flip; served from L2"},
-        "per_instance_loop": {"instances": loop_instances,
-                              "updates_per_s": loop_upd / (loop_ms * 1e-3),
+        "form": "blocked (MFMA, block-random order)",
+        "roofline": best["roofline"],
+        "forms": out,
+        "blocked_over_plain": best["value"] / out["plain"]["value"],
+        "per_instance_loop": {"instances": loop_instances, "updates_per_s": loop_rate,
                               "note": "device time (HIP events) of one plain-form engine call "
                                       "per instance, first instances of the batch"},
-        "batched_over_loop": (updates / (kms * 1e-3)) / (loop_upd / (loop_ms * 1e-3)),
+        "batched_over_loop": best["value"] / loop_rate,
     }
 
 

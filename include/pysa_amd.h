@@ -251,10 +251,19 @@ int pysa_amd_device_info(int32_t* compute_units, char* arch, int32_t arch_len);
  *               of its best states (reads*n[k] int8 at best_state + s_off[k]/R)
  * Per (instance, read) g = k*reads + read: energies/beta_idx [g*R ..],
  * best_energy/best_sweeps/accepted [g].  Local fields are exact integers
- * S_k s; the field of spin i is the float32 value scale*c_i + h_i.  Randomness
- * is the Philox stream of pysa_amd_metropolis with read index g.
- *   R <= 16 (R <= 8 when some n > 1024), order 0-2, best_semantics as in
+ * S_k s; the field of spin i is the float32 value scale*c_i + h_i (the
+ * product rounded, then the sum).  Randomness is the Philox stream of
+ * pysa_amd_metropolis with read index g.
+ *   R <= 16 (R <= 8 when some n > 1024), order 0-3, best_semantics as in
  *   pysa_amd_metropolis.
+ *   form  PYSA_AMD_FORM_PLAIN: one workgroup per (instance, read), the fields
+ *         in registers; BLOCKED or AUTO: instances with n >= 65 run the MFMA
+ *         form (int8 products of 32-spin blocks, one workgroup per instance
+ *         and up to 32 or 64 chains), the others the plain kernel.  BLOCKED
+ *         needs order 0 or 3 (PYSA_AMD_ENOTSUP otherwise); AUTO falls back
+ *         to PLAIN for orders 1-2.  Every form gives the same results for
+ *         the same order.
+ * kernel_ms: device time of the call's work (packing included).
  */
 int pysa_amd_batched_device(int32_t count, const int32_t* n, const int32_t* ld,
                             const int64_t* j_off, const int8_t* signs, const float* scale,
@@ -264,7 +273,7 @@ int pysa_amd_batched_device(int32_t count, const int32_t* n, const int32_t* ld,
                             uint64_t sweep_offset, const int64_t* s_off, int8_t* states,
                             float* energies, int64_t* beta_idx, int8_t* best_state,
                             float* best_energy, int32_t* best_sweeps, uint64_t* accepted,
-                            double* kernel_ms, void* stream);
+                            int32_t form, double* kernel_ms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -49,8 +49,8 @@ _SIGNATURES = {
     "pysa_amd_device_info": [ctypes.POINTER(_c_i32), ctypes.c_char_p, _c_i32],
     "pysa_amd_batched_device": [_c_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32,
                                 _c_i32, _c_i32, _c_i32, _c_i32, _c_i32, _vp, _c_u64, _c_u64,
-                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl),
-                                _vp],
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32,
+                                ctypes.POINTER(_c_dbl), _vp],
     "pysa_amd_malloc": [_c_u64, ctypes.POINTER(_vp)],
     "pysa_amd_free": [_vp],
     "pysa_amd_memcpy_h2d": [_vp, _vp, _c_u64],

@@ -3,8 +3,12 @@
 PySA solves a batch of problems as a Python loop over ``Solver`` objects
 (pysa/sa.py:92-430 per instance, reads one after another at
 pysa/sa.py:425-427).  ``solve_batch`` runs every read of every instance in
-one call of ``pysa_amd_batched_device``: one workgroup per (instance, read),
-largest instances first.
+one call of ``pysa_amd_batched_device``, largest instances first: with
+``form='plain'`` one workgroup per (instance, read) keeps each replica's fields
+in registers; with ``form='blocked'`` (or ``'auto'``, the default) instances of
+65 spins or more run the MFMA form, one workgroup per instance whose fields
+come from int8 products of 32-spin blocks.  Both give the same results for the
+same visit order.
 
 Instances are given as integer coupling matrices S_k (int8, symmetric, zero
 diagonal) with a float32 scale each, J_k = scale_k * S_k -- e.g. the
@@ -70,18 +74,29 @@ def initial_energies(packed, instances, states, reads, R, h=None):
 def solve_batch(instances, *, problem_type="ising", reads=4, num_replicas=8, num_sweeps=1000,
                 min_temp=0.3, max_temp=1.5, betas=None, update_strategy="random", h=None,
                 use_pt=True, parallel=True, seed=0, sweep_offset=0, states=None,
-                engine_stats=None):
+                engine_stats=None, form="auto"):
     """Run every read of every instance; returns one dict per instance with
     PySA's per-read columns (states, energies, best_state, best_energy, temps,
-    num_sweeps, min_sweeps)."""
+    num_sweeps, min_sweeps).
+
+    form: 'plain', 'blocked' or 'auto'.  With 'auto' and 'blocked',
+    update_strategy='random' is the block-random order of the MFMA form (as in
+    Solver.metropolis_update) when some instance has 65 spins or more; pass
+    form='plain' for a fresh permutation per replica and sweep."""
     if problem_type not in ("ising", "qubo"):
         raise ValueError(f"problem_type='{problem_type}' not supported.")
-    order = {"sequential": 0, "random": 1, "random_shared": 2}.get(update_strategy)
+    order = {"sequential": 0, "random": 1, "random_shared": 2,
+             "block_random": 3}.get(update_strategy)
     if order is None:
         raise ValueError(f"update_strategy='{update_strategy}' not recognized.")
+    codes = {"plain": _lib.FORM_PLAIN, "blocked": _lib.FORM_BLOCKED, "auto": _lib.FORM_AUTO}
+    if form not in codes:
+        raise ValueError(f"form='{form}' not recognized.")
     R = int(num_replicas)
     count = len(instances)
     pk = pack(instances, h)
+    if form != "plain" and order == 1 and int(pk["n"].max()) >= 65:
+        order = 3
     if betas is None:  # pysa/sa.py:167-194 (R >= 3) for every instance
         ratio = np.exp(1 / (R - 1) * np.log(max_temp / min_temp)) if R > 1 else 1.0
         lad = np.array([1 / (min_temp * ratio**k) for k in range(R)], dtype=np.float32)
@@ -112,7 +127,7 @@ def solve_batch(instances, *, problem_type="ising", reads=4, num_replicas=8, num
             dev["h_off"].ptr if "h_off" in dev else None, ptype, reads, R, int(num_sweeps), order,
             int(bool(use_pt)), 0 if parallel else 1, d_betas.ptr, int(seed) & (2**64 - 1),
             int(sweep_offset), d_soff.ptr, d_st.ptr, d_en.ptr, d_bi.ptr, d_bs.ptr, d_be.ptr,
-            d_bw.ptr, d_acc.ptr, _lib.ctypes.byref(kms), None))
+            d_bw.ptr, d_acc.ptr, codes[form], _lib.ctypes.byref(kms), None))
         st, ene, bi = d_st.numpy(), d_en.numpy(), d_bi.numpy()
         bs, be, bw, acc = d_bs.numpy(), d_be.numpy(), d_bw.numpy(), d_acc.numpy()
     finally:

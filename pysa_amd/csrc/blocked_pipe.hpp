@@ -72,12 +72,36 @@ struct __align__(16) PipeLds {
 // BR: block-random visit order (schedule tables in LDS); otherwise blocks are
 // visited in order with no in-block shuffle, and every schedule entry is
 // plain arithmetic on the step.
-template <int NP, bool I8, int NW, int MW, bool BR, int NT>
-__global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p) {
+// BAT: the batched form.  Each workgroup runs one instance of a batch (its
+// entry of p.bat), with couplings scale * S (S int8); the fields are kept as
+// the exact integer counts S s, and the field of a spin is the f32 value
+// scale * c + h, rounded twice like the batched chain of the oracle; the
+// energy changes are summed in f64 in visit order.
+template <int NP, bool I8, int NW, int MW, bool BR, int NT, bool BAT = false>
+__global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0) {
+  static_assert(!BAT || (I8 && NP == 1), "the batched form multiplies int8 couplings");
   __shared__ PipeLds<MW> L;
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tid = threadIdx.x;
+  BParams p = p0;
+  int read0 = blockIdx.x * p0.rpw, read_end = p0.reads;
+  int64_t spin_adj = 0, best_adj = 0;  // batched: see BatWG
+  if constexpr (BAT) {
+    const BatWG& bw = p0.bat[blockIdx.x];
+    p.n = bw.n;
+    p.nblk = bw.nblk;
+    p.kb = bw.kb;
+    p.ld = bw.ld;
+    p.scale = bw.scale;
+    p.frag = reinterpret_cast<const int8_t*>(p0.frag) + bw.frag;
+    p.sgn = p0.sgn + bw.sgn;
+    p.h = bw.h >= 0 ? p0.h + bw.h : nullptr;
+    p.betas = p0.betas + (size_t)bw.inst * p0.R;
+    read0 = bw.read0;
+    read_end = bw.read_end;
+    spin_adj = bw.spin_adj;
+    best_adj = bw.best_adj;
+  }
   const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
-  const int read0 = blockIdx.x * p.rpw;
   const int h = lane >> 5;
   // Two waves per SIMD (SIMD w & 3).  Wave 0 runs the proposals; matrix
   // wave w = 1..MW computes the products of K range m = w - 1 for both tiles
@@ -99,14 +123,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   for (int tt = 0; tt < NT; ++tt) {
     qtt[tt] = tt * 32 + (lane & 31);
     gtt[tt] = qtt[tt] / R;
-    actt[tt] = mw && (gtt[tt] < p.rpw) && (read0 + gtt[tt] < p.reads);
-    base_t[tt] = actt[tt] ? ((size_t)(read0 + gtt[tt]) * R + (qtt[tt] - gtt[tt] * R)) * n : 0;
+    actt[tt] = mw && (gtt[tt] < p.rpw) && (read0 + gtt[tt] < read_end);
+    base_t[tt] = actt[tt] ? (size_t)(spin_adj + ((int64_t)(read0 + gtt[tt]) * R +
+                                                 (qtt[tt] - gtt[tt] * R)) * n)
+                          : 0;
   }
   // Replica role (every wave): lane = replica q of the group.
   const int q = lane;
   const int g = q / R, rr = q - g * R;
   const int read = read0 + g;
-  const bool act = (g < p.rpw) && (read < p.reads);
+  const bool act = (g < p.rpw) && (read < read_end);
   const uint64_t cid = (uint64_t)read * R + rr;
   const int c0 = mw ? m * kb : 0;
   const int jn = mw ? max(0, min(kb, nblk - c0)) : 0;
@@ -207,11 +233,14 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
       const int ri = i ^ (int)xm, rk = k ^ (int)xm;
       const bool ok = e >= 0 && e < kB * kB && k < kB && pos0 + ri < n && pos0 + rk < n;
       jb_ok |= (uint32_t)ok << e2;
-      jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
+      if constexpr (BAT)  // S is symmetric: S^T = S
+        jb_reg[e2] = (float)p.sgn[ok ? (size_t)(pos0 + ri) * p.ld + pos0 + rk : 0];
+      else
+        jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
     }
     const bool hok = hl >= 0 && hl < kB && pos0 + hl < n;
     jb_ok |= (uint32_t)hok << 31;
-    h_reg = p.h[hok ? pos0 + hl : 0];
+    h_reg = (BAT && !p.h) ? 0.f : p.h[hok ? pos0 + hl : 0];
     if (!I8 && p.rowsum) rs_reg = p.rowsum[hok ? pos0 + hl : 0];
   };
   auto publish_block = [&](int pb) {
@@ -442,6 +471,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
   // rsb (zero otherwise); fsc is a power of two, so the fma rounds like the add
   const bool qsum = !I8 && p.ptype == 1;
   const float fsc = qsum ? 0.5f : 1.f;
+  const float bsc = BAT ? p.scale : 1.f;  // batched: field = bsc * count + h
   double eoff = 0.0;  // wave 0: caller's initial energy minus its recomputation
 
   // The correction J[c1, c] . S_c of the next block c1 by the proposal wave,
@@ -547,7 +577,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           hib[4 * t4] = a.x, hib[4 * t4 + 1] = a.y, hib[4 * t4 + 2] = a.z, hib[4 * t4 + 3] = a.w;
           lob[4 * t4] = b.x, lob[4 * t4 + 1] = b.y, lob[4 * t4 + 2] = b.z, lob[4 * t4 + 3] = b.w;
         }
-        float fp[kB];
+        // fields in visit order (batched: the integer counts, with h in hk)
+        float fp[kB], hk[kB];
         {
           const int xg = (int)(xm >> 2);
           // visit group r4 holds block rows 4 (r4 ^ xg) + (e ^ (xm & 3)); the
@@ -586,8 +617,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
                                      sm.z + cr.z + s4.z, sm.w + cr.w + s4.w};
                 const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                  fp[4 * (g0 + j) + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
+                for (int e = 0; e < 4; ++e) {
+                  if constexpr (BAT) {
+                    fp[4 * (g0 + j) + e] = vs[e ^ P];
+                    hk[4 * (g0 + j) + e] =
+                        __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(hv[e ^ P])));
+                  } else {
+                    fp[4 * (g0 + j) + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
+                  }
+                }
               }
               __builtin_amdgcn_sched_barrier(0);
             }
@@ -610,6 +648,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
         uint32_t vsgn = 0x80000000u;  // kept in a register (not a literal operand)
         asm volatile("" : "+v"(vsgn));
         float dpart = 0.f;
+        const double dsum0 = dsum;  // batched: the sum before the block (for a rerun)
         uint64_t amb = 0;  // lanes with a decision inside its bracket
         // Rows of J[blk][blk] (the same for every lane) stream from LDS
         // through a ring JR rows ahead of the chain.  Row k holds only the
@@ -630,7 +669,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           // dn = 2 s (F + h) (ising) or (2x - 1)(F + h) (qubo), exact in f32
           const uint32_t shk = nmb << (31 - k);
           const float sg = __uint_as_float((shk & vsgn) | (cub & ~vsgn));  // one bit insert
-          const float dn = fp[k] * sg;
+          // batched: the field scale * c + h of the count (a product, then a sum)
+          const float fk = BAT ? fp[k] * bsc + hk[k] : fp[k];
+          const float dn = fk * sg;
           const float hik = hib[k], lok = lob[k];
           // idle lanes and padding rows have lo = hi = +inf (and finite dn):
           // never taken, never inside their bracket
@@ -645,8 +686,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(tk), "=s"(co) : "v"(tk), "s"(tm));
             (void)co;
           }
-          // dS * fp[k] is exactly -dn or 0, so this rounds like dpart - dn
-          dpart = __builtin_fmaf(dS, fp[k], dpart);
+          // dS * fk is exactly -dn or 0, so this rounds like dpart - dn
+          // (batched: summed in f64 in visit order, as the oracle sums)
+          if constexpr (BAT)
+            dsum += (double)(dS * fk);
+          else
+            dpart = __builtin_fmaf(dS, fp[k], dpart);
           static_for<(kB - 1 - k + 3) / 4>([&](auto tc) __attribute__((always_inline)) {
             constexpr int t4 = decltype(tc)::value;
             const float4 j4 = jring[k % JR][t4];
@@ -675,14 +720,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             float v = L.part[pb][0][fld(qo, br)];
             for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(qo, br)];
             v = v + L.corr[fld(qo, br)];
-            sf[r] = __builtin_fmaf(fsc, v + L.rsb[pb][br], L.hb[pb][br]);
+            sf[r] = BAT ? v : __builtin_fmaf(fsc, v + L.rsb[pb][br], L.hb[pb][br]);
           }
           mb = mb0;
           dpart = 0.f;
+          if constexpr (BAT) dsum = dsum0;
           for (int k = 0; k < kB; ++k) {
             const bool up = (mb >> k) & 1u;
             const float sg = up ? cu : -cu;
-            const float dn = sf[k] * sg;
+            const float fk = BAT ? sf[k] * bsc + L.hb[pb][k ^ (int)xm] : sf[k];
+            const float dn = fk * sg;
             // idle lanes and padding rows have lo = hi = +inf: never taken
             bool take = act & (dn > L.hi[pb][qo][k]);
             if (act && !take && !(dn < L.lo[pb][qo][k])) {
@@ -691,7 +738,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
               take = accept_exact(wd, beta, dn);
             }
             mb ^= (uint32_t)take << k;
-            dpart += take ? -dn : 0.f;
+            if constexpr (BAT)
+              dsum += take ? (double)(-dn) : 0.0;
+            else
+              dpart += take ? -dn : 0.f;
             const float dS = take ? -sg : 0.f;
             for (int r = k + 1; r < kB; ++r)
               sf[r] = __builtin_fmaf(L.jb[pb][k * kB + (r - k - 1)], dS, sf[r]);
@@ -705,7 +755,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
           if (c1 == c) L.bb[pb ^ 1][qo] = mbr;
           corr_products(fcorr, mbr, qo);
         }
-        dsum += (double)dpart;
+        if constexpr (!BAT) dsum += (double)dpart;
         nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
         __syncthreads();  // B
         if (lastb) {
@@ -827,7 +877,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
             for (int tt = 0; tt < NT; ++tt)
               if (mw && p.best_sem == 1 && actt[tt] && L.src[gtt[tt]] == qtt[tt] - gtt[tt] * R)
                 store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype,
-                                    p.best_state + (size_t)(read0 + gtt[tt]) * n);
+                                    p.best_state + (size_t)(best_adj + (int64_t)(read0 + gtt[tt]) * n));
           }
           // the next pass's first block (after the exchanges: new betas)
           if (hw && w > MW + 1 && !dry_next) brackets(c1, m1, sw_next, pn);
@@ -861,17 +911,17 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p)
     for (int tt = 0; tt < NT; ++tt)
       if (actt[tt] && qtt[tt] - gtt[tt] * R == 0 && !L.src[gtt[tt]])
         store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype,
-                            p.best_state + (size_t)(read0 + gtt[tt]) * n);
+                            p.best_state + (size_t)(best_adj + (int64_t)(read0 + gtt[tt]) * n));
   }
 }
 
-template <int NP, bool I8, bool BR, int NT>
+template <int NP, bool I8, bool BR, int NT, bool BAT = false>
 hipError_t launch_pipe_o(const BParams& p, int nw, int groups, hipStream_t st) {
   dim3 grid(groups), block(128 * (kPipeMW + 1));
   switch (nw) {
-    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW, BR, NT>), grid, block, 0, st, p); break;
-    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW, BR, NT>), grid, block, 0, st, p); break;
-    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW, BR, NT>), grid, block, 0, st, p); break;
+    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW, BR, NT, BAT>), grid, block, 0, st, p); break;
+    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW, BR, NT, BAT>), grid, block, 0, st, p); break;
+    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW, BR, NT, BAT>), grid, block, 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -879,14 +929,14 @@ hipError_t launch_pipe_o(const BParams& p, int nw, int groups, hipStream_t st) {
 
 // Launch of one coupling encoding: nw spin words per matrix wave, nt tiles of
 // 32 replicas per workgroup, the order from p.
-template <int NP, bool I8>
+template <int NP, bool I8, bool BAT = false>
 hipError_t launch_pipe_enc(const BParams& p, int nw, int nt, int groups, hipStream_t st) {
   if (nt != 1 && nt != 2) return hipErrorInvalidValue;
   if (p.order == kBlockRandom)
-    return nt == 2 ? launch_pipe_o<NP, I8, true, 2>(p, nw, groups, st)
-                   : launch_pipe_o<NP, I8, true, 1>(p, nw, groups, st);
-  return nt == 2 ? launch_pipe_o<NP, I8, false, 2>(p, nw, groups, st)
-                 : launch_pipe_o<NP, I8, false, 1>(p, nw, groups, st);
+    return nt == 2 ? launch_pipe_o<NP, I8, true, 2, BAT>(p, nw, groups, st)
+                   : launch_pipe_o<NP, I8, true, 1, BAT>(p, nw, groups, st);
+  return nt == 2 ? launch_pipe_o<NP, I8, false, 2, BAT>(p, nw, groups, st)
+                 : launch_pipe_o<NP, I8, false, 1, BAT>(p, nw, groups, st);
 }
 
 // One per translation unit (blocked_pipe_i8.hip, blocked_pipe_bf16_{1,2,3}.hip).
@@ -894,6 +944,8 @@ hipError_t launch_pipe_i8(const BParams& p, int nw, int nt, int groups, hipStrea
 hipError_t launch_pipe_bf16_1(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 hipError_t launch_pipe_bf16_2(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 hipError_t launch_pipe_bf16_3(const BParams& p, int nw, int nt, int groups, hipStream_t st);
+// The batched form (blocked_pipe_bat.hip): one workgroup per entry of p.bat.
+hipError_t launch_pipe_bat(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 
 }  // namespace blk
 }  // namespace pysa

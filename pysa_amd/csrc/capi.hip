@@ -787,23 +787,31 @@ extern "C" int pysa_amd_batched_device(int32_t count, const int32_t* n, const in
                                        const float* betas, uint64_t seed, uint64_t sweep_offset,
                                        const int64_t* s_off, int8_t* states, float* energies,
                                        int64_t* beta_idx, int8_t* best_state, float* best_energy,
-                                       int32_t* best_sweeps, uint64_t* accepted, double* kernel_ms,
-                                       void* stream) {
+                                       int32_t* best_sweeps, uint64_t* accepted, int32_t form,
+                                       double* kernel_ms, void* stream) {
   g_err.clear();
   if (count < 1 || reads < 1 || R < 1 || R > 16 || S < 1 || order < 0 || order > 3 ||
       problem_type < 0 || problem_type > 1 || best_semantics < 0 || best_semantics > 1)
     return fail(PYSA_AMD_EINVAL, "batched: count, reads >= 1, 1 <= R <= 16, S >= 1, order 0-3");
+  if (form != PYSA_AMD_FORM_PLAIN && form != PYSA_AMD_FORM_BLOCKED && form != PYSA_AMD_FORM_AUTO)
+    return fail(PYSA_AMD_EINVAL, "batched: form must be PLAIN, BLOCKED or AUTO");
+  const bool mfma_order = order == pysa::kSequential || order == pysa::kBlockRandom;
+  if (form == PYSA_AMD_FORM_BLOCKED && !mfma_order)
+    return fail(PYSA_AMD_ENOTSUP, "batched MFMA form: sequential or block-random order only");
   if (!n || !ld || !j_off || !signs || !scale || !betas || !s_off || !states || !energies ||
       !beta_idx || !best_state || !best_energy || !best_sweeps || (h && !h_off))
     return fail(PYSA_AMD_EINVAL, "batched: null array");
   if ((int64_t)count * reads > 0x7fffffffLL) return fail(PYSA_AMD_EINVAL, "batched: too many reads");
   hipStream_t st = (hipStream_t)stream;
   std::vector<int32_t> hn(count), hld(count);
-  std::vector<int64_t> hj(count), hs(count);
+  std::vector<int64_t> hj(count), hs(count), hho(h ? count : 0);
+  std::vector<float> hsc(count);
   HIP_TRY(hipMemcpyAsync(hn.data(), n, sizeof(int32_t) * count, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(hld.data(), ld, sizeof(int32_t) * count, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(hj.data(), j_off, sizeof(int64_t) * count, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(hs.data(), s_off, sizeof(int64_t) * count, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hsc.data(), scale, sizeof(float) * count, hipMemcpyDeviceToHost, st));
+  if (h) HIP_TRY(hipMemcpyAsync(hho.data(), h_off, sizeof(int64_t) * count, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   int max_n = 0;
   int64_t acc_states = 0;
@@ -818,15 +826,21 @@ extern "C" int pysa_amd_batched_device(int32_t count, const int32_t* n, const in
   }
   if (max_n > 1024 && R > 8) return fail(PYSA_AMD_EINVAL, "batched: R <= 8 when an instance has n > 1024");
   // Largest instances first, so the short ones fill the tail of the launch.
-  const int items = count * reads;
-  std::vector<int32_t> om(items);
-  std::vector<int> idx(count);
+  // The MFMA form (BLOCKED / AUTO, sequential or block-random order) takes
+  // the instances of three blocks or more; the plain kernel the others.
+  std::vector<int> idx(count), mf, pl;
   for (int k = 0; k < count; ++k) idx[k] = k;
   std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return hn[a] > hn[b]; });
-  for (int q = 0, o = 0; q < count; ++q)
-    for (int r = 0; r < reads; ++r) om[o++] = idx[q] * reads + r;
+  const bool use_mfma = form != PYSA_AMD_FORM_PLAIN && mfma_order;
+  for (int k : idx) (use_mfma && hn[k] >= pysa::kBatchedMfmaMinN ? mf : pl).push_back(k);
+  int max_n_pl = 0;
+  for (int k : pl) max_n_pl = hn[k] > max_n_pl ? hn[k] : max_n_pl;
+  const int items = (int)pl.size() * reads;
+  std::vector<int32_t> om(items > 0 ? items : 1);
+  for (int q = 0, o = 0; q < (int)pl.size(); ++q)
+    for (int r = 0; r < reads; ++r) om[o++] = pl[q] * reads + r;
   void *d_om = nullptr, *d_best = nullptr;
-  HIP_TRY(hipMalloc(&d_om, sizeof(int32_t) * items));
+  HIP_TRY(hipMalloc(&d_om, sizeof(int32_t) * om.size()));
   if (best_semantics == 0 && hipMalloc(&d_best, (size_t)acc_states) != hipSuccess) {
     (void)hipFree(d_om);
     return fail(PYSA_AMD_EHIP, "batched: allocating best-state scratch");
@@ -839,8 +853,11 @@ extern "C" int pysa_amd_batched_device(int32_t count, const int32_t* n, const in
       if (b) (void)hipFree(b);
     }
   } frees{d_om, d_best};
-  HIP_TRY(hipMemcpyAsync(d_om, om.data(), sizeof(int32_t) * items, hipMemcpyHostToDevice, st));
-  if (accepted) HIP_TRY(hipMemsetAsync(accepted, 0, sizeof(uint64_t) * items, st));
+  HIP_TRY(hipMemcpyAsync(d_om, om.data(), sizeof(int32_t) * om.size(), hipMemcpyHostToDevice, st));
+  if (accepted) HIP_TRY(hipMemsetAsync(accepted, 0, sizeof(uint64_t) * count * reads, st));
+  // the MFMA kernel's per-replica bests start from the initial states
+  if (best_semantics == 0 && !mf.empty())
+    HIP_TRY(hipMemcpyAsync(d_best, states, (size_t)acc_states, hipMemcpyDeviceToDevice, st));
   pysa::BatchParams p;
   p.count = count;
   p.reads = reads;
@@ -871,12 +888,19 @@ extern "C" int pysa_amd_batched_device(int32_t count, const int32_t* n, const in
   p.best_sweeps = best_sweeps;
   p.accepted = (unsigned long long*)accepted;
   p.best_rep = (int8_t*)d_best;
+  pysa::BatchedMfmaRun mrun;  // its buffers are freed after the synchronisation below
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(PYSA_AMD_EHIP, "batched: creating events");
+  }
   (void)hipEventRecord(e0, st);
-  hipError_t le = pysa::launch_batched(p, items, max_n, st);
+  hipError_t le = mrun.enqueue(p, mf, hn.data(), hld.data(), hj.data(), hs.data(),
+                               h ? hho.data() : nullptr, hsc.data(), st);
+  if (le == hipSuccess && items > 0) le = pysa::launch_batched(p, items, max_n_pl, st);
   (void)hipEventRecord(e1, st);
+  if (le != hipSuccess) (void)hipStreamSynchronize(st);  // nothing in flight when buffers are freed
   hipError_t se = (le == hipSuccess) ? hipEventSynchronize(e1) : le;
   float t = 0.f;
   if (se == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace pysa {
 
 // Visit orders: sequential; a fresh Feistel permutation per chain and sweep;
@@ -112,5 +114,48 @@ struct BatchParams {
 };
 constexpr int kBatchedMaxN = 2048;
 hipError_t launch_batched(const BatchParams& p, int items, int max_n, hipStream_t st);
+
+// ---- batched MFMA form (blocked_pipe_bat.hip) ----
+// One workgroup of the batched MFMA form: the instance it runs and where
+// that instance's data lie.  States of global read g, replica r start at
+// spin_adj + (g R + r) n (spin_adj = the instance's state offset minus the
+// global index of its first chain times n), its best state at
+// best_adj + g n; the offsets are never negative once a chain's own term is
+// added.
+struct BatWG {
+  int n, nblk, kb, ld;  // spins, 32-spin blocks, K blocks per matrix wave, row stride of S
+  int inst, read0, read_end;  // instance; first global read of the group; end of its reads
+  float scale;
+  int64_t frag;      // byte offset of the instance's int8 MFMA fragments
+  int64_t sgn;       // offset of its int8 coupling matrix S
+  int64_t h;         // offset of its local fields, -1 when h = 0
+  int64_t spin_adj;  // see above
+  int64_t best_adj;
+};
+// One instance of the fragment packing.
+struct BatPack {
+  int n, nblk, ld, pad;
+  int64_t sgn, frag;
+};
+// Smallest instance of the batched MFMA form (three blocks, one per matrix wave).
+constexpr int kBatchedMfmaMinN = 65;
+// Enqueues the instances `inst` (indices into the batch, largest first) of a
+// batch on the pipelined MFMA kernel: packs their couplings, builds the
+// workgroup table and launches.  Host copies of the instance arrays are
+// passed in (scale, h_off: null when h is).  Owns the device buffers until
+// destroyed, which must follow the stream's completion.  best_rep is the
+// batch's state-layout scratch holding the initial states (best_sem 0).
+struct BatchedMfmaRun {
+  void* frag = nullptr;
+  void* tab = nullptr;
+  void* pack = nullptr;
+  void* best_es = nullptr;
+  std::vector<BatWG> wg;
+  std::vector<BatPack> pk;
+  ~BatchedMfmaRun();
+  hipError_t enqueue(const BatchParams& bp, const std::vector<int>& inst, const int32_t* hn,
+                     const int32_t* hld, const int64_t* hj, const int64_t* hs,
+                     const int64_t* hh_off, const float* hscale, hipStream_t st);
+};
 
 }  // namespace pysa
