@@ -154,13 +154,11 @@ struct BParams {
   int* best_sweeps;     // reads
   double* log_omegas;   // reads*S or null
   unsigned long long* accepted;
-  // batched MFMA form (pipelined kernel with BAT): one entry per workgroup;
-  // couplings scale * S with S int8 (sgn, rows of ld bytes), fields the f32
-  // value scale * (S s) + h
+  // batched MFMA form (pipelined kernel with BAT): one entry per workgroup,
+  // the int8 coupling matrices of the batch (offsets in the entries); h,
+  // betas and frag are then the batch's base pointers
   const BatWG* bat;
   const int8_t* sgn;
-  int ld;
-  float scale;
 };
 
 // Store the spins of one operand-role replica for the blocks this wave owns.

@@ -43,13 +43,15 @@ __device__ __forceinline__ int fld(int q, int r) {
 }
 constexpr int kHL = 36;
 constexpr int kFG = 2;  // row groups per stage of the proposal wave's field sums
-template <int MW>
+// NI: instances per workgroup (the batched form keeps one J block and h per
+// tile).
+template <int MW, int NI>
 struct __align__(16) PipeLds {
   float part[2][MW][64 * 32];  // [parity][matrix wave][fld(replica, row)]
   float corr[64 * 32];         // J[c+1, c] . S_c of the next block
-  float jb[2][kB * kB];       // row i of J[blk][blk] right of the diagonal, shifted left:
+  float jb[2][NI][kB * kB];   // row i of J[blk][blk] right of the diagonal, shifted left:
                               // J^T[pos0+i][pos0+i+1+t] at i*32 + t (t < 31 - i), else 0
-  float hb[2][kB];
+  float hb[2][NI][kB];
   float rsb[2][kB];
   // acceptance brackets [parity][replica][visit index], rows padded to
   // kHL floats so that 16-byte reads of consecutive replicas hit distinct banks
@@ -72,36 +74,36 @@ struct __align__(16) PipeLds {
 // BR: block-random visit order (schedule tables in LDS); otherwise blocks are
 // visited in order with no in-block shuffle, and every schedule entry is
 // plain arithmetic on the step.
-// BAT: the batched form.  Each workgroup runs one instance of a batch (its
-// entry of p.bat), with couplings scale * S (S int8); the fields are kept as
-// the exact integer counts S s, and the field of a spin is the f32 value
-// scale * c + h, rounded twice like the batched chain of the oracle; the
-// energy changes are summed in f64 in visit order.
+// BAT: the batched form.  Each workgroup runs two tiles (its entry of
+// p.bat), each up to 32 chains of one instance of a batch, the two instances
+// with the same number of blocks; couplings are scale * S (S int8), the
+// fields kept as the exact integer counts S s, and the field of a spin is the
+// f32 value scale * c + h, rounded twice like the batched chain of the
+// oracle; the energy changes are summed in f64 in visit order.
 template <int NP, bool I8, int NW, int MW, bool BR, int NT, bool BAT = false>
 __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0) {
-  static_assert(!BAT || (I8 && NP == 1), "the batched form multiplies int8 couplings");
-  __shared__ PipeLds<MW> L;
-  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tid = threadIdx.x;
+  // a local copy (the register allocation of the block loops was found to be
+  // sensitive to this; the copy keeps it as measured)
   BParams p = p0;
-  int read0 = blockIdx.x * p0.rpw, read_end = p0.reads;
-  int64_t spin_adj = 0, best_adj = 0;  // batched: see BatWG
+  static_assert(!BAT || (I8 && NP == 1 && NT == 2),
+                "the batched form multiplies int8 couplings, two tiles per workgroup");
+  constexpr int NI = BAT ? NT : 1;  // instances per workgroup (coupling sets)
+  __shared__ PipeLds<MW, NI> L;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), tid = threadIdx.x;
+  const int read0 = blockIdx.x * p.rpw;
+  const int R = p.R, n = p.n;
+  const int nblk = BAT ? p.bat[blockIdx.x].nblk : p.nblk;
+  const int kb = BAT ? p.bat[blockIdx.x].kb : p.kb;
+  // batched: tile tt's instance and reads, read once at the start (before
+  // any store, so they are scalar loads kept in registers; a load later in
+  // the kernel could not be proven unclobbered and would be a vector load)
+  BatTile bt[NI];
   if constexpr (BAT) {
-    const BatWG& bw = p0.bat[blockIdx.x];
-    p.n = bw.n;
-    p.nblk = bw.nblk;
-    p.kb = bw.kb;
-    p.ld = bw.ld;
-    p.scale = bw.scale;
-    p.frag = reinterpret_cast<const int8_t*>(p0.frag) + bw.frag;
-    p.sgn = p0.sgn + bw.sgn;
-    p.h = bw.h >= 0 ? p0.h + bw.h : nullptr;
-    p.betas = p0.betas + (size_t)bw.inst * p0.R;
-    read0 = bw.read0;
-    read_end = bw.read_end;
-    spin_adj = bw.spin_adj;
-    best_adj = bw.best_adj;
+#pragma unroll
+    for (int tt = 0; tt < NI; ++tt) bt[tt] = p.bat[blockIdx.x].t[tt];
   }
-  const int R = p.R, n = p.n, nblk = p.nblk, kb = p.kb;
+  auto tile = [&](int tt) -> const BatTile& { return bt[tt]; };
+  auto n_of = [&](int tt) { return BAT ? tile(tt).n : n; };
   const int h = lane >> 5;
   // Two waves per SIMD (SIMD w & 3).  Wave 0 runs the proposals; matrix
   // wave w = 1..MW computes the products of K range m = w - 1 for both tiles
@@ -116,23 +118,52 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   const int hl = hw ? (w - MW - 1) * 64 + lane : -1;  // helper-lane index 0..64 (MW+1) - 1
   // Operand role (matrix waves): replica qtt[tt] of tile tt.
   static_assert(NT == 1 || NT == 2, "one or two tiles of 32 replicas");
+  // gtt: its read within the workgroup (recomputed where needed: kept in
+  // registers across the block loop they would add to its pressure)
   int qtt[NT], gtt[NT];
   bool actt[NT];
   size_t base_t[NT];
+  auto rtt_of = [&](int tt) {  // its replica within the read
+    return BAT ? (lane & 31) % R : qtt[tt] - gtt[tt] * R;
+  };
+  auto rdt_of = [&](int tt) {  // the read's global index
+    return BAT ? tile(tt).read0 + (lane & 31) / R : read0 + gtt[tt];
+  };
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt) {
     qtt[tt] = tt * 32 + (lane & 31);
-    gtt[tt] = qtt[tt] / R;
-    actt[tt] = mw && (gtt[tt] < p.rpw) && (read0 + gtt[tt] < read_end);
-    base_t[tt] = actt[tt] ? (size_t)(spin_adj + ((int64_t)(read0 + gtt[tt]) * R +
-                                                 (qtt[tt] - gtt[tt] * R)) * n)
-                          : 0;
+    if constexpr (BAT) {  // tile tt: p.rpw reads of its own instance
+      const int gl = (lane & 31) / R;
+      gtt[tt] = tt * p.rpw + gl;
+      actt[tt] = mw && gl < p.rpw && rdt_of(tt) < tile(tt).read_end;
+      base_t[tt] = actt[tt] ? (size_t)(tile(tt).spin_adj +
+                                       ((int64_t)rdt_of(tt) * R + rtt_of(tt)) * tile(tt).n)
+                            : 0;
+    } else {  // p.rpw reads over the tiles
+      gtt[tt] = qtt[tt] / R;
+      actt[tt] = mw && (gtt[tt] < p.rpw) && (read0 + gtt[tt] < p.reads);
+      base_t[tt] = actt[tt] ? ((size_t)(read0 + gtt[tt]) * R + (qtt[tt] - gtt[tt] * R)) * n : 0;
+    }
   }
-  // Replica role (every wave): lane = replica q of the group.
+  // the best state of tile tt's read (operand role)
+  auto best_of = [&](int tt) -> int8_t* {
+    if constexpr (BAT) return p.best_state + (tile(tt).best_adj + (int64_t)rdt_of(tt) * tile(tt).n);
+    return p.best_state + (size_t)(read0 + gtt[tt]) * n;
+  };
+  // Replica role (every wave): lane = replica q of the group; g its read
+  // within the workgroup, tq its tile in the batched form (whose instance's
+  // size, scale and ladder the lane uses).
   const int q = lane;
-  const int g = q / R, rr = q - g * R;
-  const int read = read0 + g;
-  const bool act = (g < p.rpw) && (read < read_end);
+  const int tq = BAT ? (q >> 5) : 0;
+  const int gl = BAT ? (q & 31) / R : 0;  // batched: its read within the tile
+  const int g = BAT ? tq * p.rpw + gl : q / R;
+  const int rr = BAT ? (q & 31) - gl * R : q - g * R;
+  const int read = BAT ? (tq ? tile(1).read0 : tile(0).read0) + gl : read0 + g;
+  const bool act = BAT ? gl < p.rpw && read < (tq ? tile(1).read_end : tile(0).read_end)
+                       : (g < p.rpw) && (read < p.reads);
+  const int nq = BAT ? (tq ? tile(1).n : tile(0).n) : n;
+  const float* betas_q =
+      BAT ? p.betas + (size_t)(tq ? tile(1).inst : tile(0).inst) * R : p.betas;
   const uint64_t cid = (uint64_t)read * R + rr;
   const int c0 = mw ? m * kb : 0;
   const int jn = mw ? max(0, min(kb, nblk - c0)) : 0;
@@ -148,7 +179,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
       for (int b = 0; b < 16; ++b) {
         const int rel = I8 ? (16 * h + b) : ((b < 8) ? (8 * h + b) : (16 + 8 * h + (b - 8)));
         const int pos = 32 * c + rel;
-        if (pos < n && actt[tt] && p.spins[base_t[tt] + pos] == 1) va |= 1u << b;
+        if (pos < n_of(tt) && actt[tt] && p.spins[base_t[tt] + pos] == 1) va |= 1u << b;
       }
       set_half<NW>(sa[tt], j, va);
     }
@@ -159,7 +190,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   if (w == 0) {
     E = act ? p.energies[(size_t)read * R + rr] : 0.f;
     L.bi[q] = act ? rr : 0;  // beta_idx = arange(R) (sa.py:325)
-    L.rb[q] = act ? p.betas[rr] : 0.f;
+    L.rb[q] = act ? betas_q[rr] : 0.f;
     bestE = E;
     gE = E;  // simulation.py:205 (meaningful on read leaders, rr == 0)
     if (p.best_sem == 0 && act) {
@@ -182,13 +213,18 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   constexpr int NHL = 64 * (MW + 1);  // helper lanes
   constexpr int NJB = (kB * kB + NHL - 1) / NHL;
   typedef typename std::conditional<I8, i32x4, bf16x8>::type Frag;
-  Frag ring[PF][NST][NP];
-  float jb_reg[NJB], h_reg = 0.f, rs_reg = 0.f;
-  uint32_t jb_ok = 0;  // bit e2: jb_reg[e2] is a coupling; bit 31: h_reg / rs_reg are
-  auto frag_ptr = [&](int c, int j, int st, int pc) -> const Frag* {
-    if (I8) return reinterpret_cast<const Frag*>(p.frag) + ((size_t)c * nblk + c0 + j) * 64 + lane;
-    return reinterpret_cast<const Frag*>(p.frag) +
-           (((size_t)c * (2 * nblk) + 2 * (c0 + j) + st) * NP + pc) * 64 + lane;
+  Frag ring[NI][PF][NST][NP];  // by instance (tile in the batched form)
+  float jb_reg[NI][NJB], h_reg[NI] = {}, rs_reg = 0.f;
+  uint32_t jb_ok[NI] = {};  // bit e2: jb_reg[.][e2] is a coupling; bit 31: h_reg / rs_reg are
+  // the coupling fragments of instance ti
+  auto frag_base = [&](int ti) -> const Frag* {
+    if constexpr (BAT)
+      return reinterpret_cast<const Frag*>(reinterpret_cast<const int8_t*>(p.frag) + tile(ti).frag);
+    return reinterpret_cast<const Frag*>(p.frag);
+  };
+  auto frag_ptr = [&](int ti, int c, int j, int st, int pc) -> const Frag* {
+    if (I8) return frag_base(ti) + ((size_t)c * nblk + c0 + j) * 64 + lane;
+    return frag_base(ti) + (((size_t)c * (2 * nblk) + 2 * (c0 + j) + st) * NP + pc) * 64 + lane;
   };
   // Stage j of this wave's K range; stages past the range reload the
   // range's last K block (recently read, so a cache hit), so every path
@@ -196,7 +232,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   // would make the compiler wait for all of them).
   // One K block of fragments: NST * NP * 64 of them, consecutive.
   constexpr int kStage = NST * NP * 64;
-  auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP]) {
+  auto load_stage = [&](int c, int j, Frag (&dst)[NST][NP], int ti) {
     // jn passes through an empty asm so that the per-stage offsets are
     // recomputed (a few scalar ops) rather than hoisted out of the block
     // loop into registers that would spill
@@ -204,7 +240,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
     asm volatile("" : "+s"(jnl));
     const int jc = jnl > 0 ? min(j, jnl - 1) : nblk - 1 - c0;
     // wave-uniform base of the stage, then a per-lane 16-byte offset
-    const Frag* base = frag_ptr(c, 0, 0, 0) - lane + (size_t)jc * kStage;
+    const Frag* base = frag_ptr(ti, c, 0, 0, 0) - lane + (size_t)jc * kStage;
 #pragma unroll
     for (int st = 0; st < NST; ++st)
 #pragma unroll
@@ -214,7 +250,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   // (matrix waves).
   auto prefetch_frags = [&](int c) {
 #pragma unroll
-    for (int u = 0; u < PF; ++u) load_stage(c, u, ring[u]);
+    for (int u = 0; u < PF; ++u)
+#pragma unroll
+      for (int ti = 0; ti < NI; ++ti) load_stage(c, u, ring[ti][u], ti);
   };
   // Request row block c, visited with in-block mask xm, spread over the
   // helper lanes: J[blk][blk] in visit order, h and row sums.
@@ -223,36 +261,57 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
     // Every lane loads (an in-bounds address when its element is padding)
     // and publish_block zeroes the padding: no register is written here
     // except by these loads, which keeps them from waiting on older loads.
-    jb_ok = 0;
+    if constexpr (!BAT) {
+      jb_ok[0] = 0;
 #pragma unroll
-    for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = hl + e2 * NHL;
-      // slot t = e & 31 of visit row i holds visit column k = i + 1 + t;
-      // visit index v is block row v ^ xm
-      const int i = e >> 5, k = i + 1 + (e & 31);
-      const int ri = i ^ (int)xm, rk = k ^ (int)xm;
-      const bool ok = e >= 0 && e < kB * kB && k < kB && pos0 + ri < n && pos0 + rk < n;
-      jb_ok |= (uint32_t)ok << e2;
-      if constexpr (BAT)  // S is symmetric: S^T = S
-        jb_reg[e2] = (float)p.sgn[ok ? (size_t)(pos0 + ri) * p.ld + pos0 + rk : 0];
-      else
-        jb_reg[e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
+      for (int e2 = 0; e2 < NJB; ++e2) {
+        const int e = hl + e2 * NHL;
+        // slot t = e & 31 of visit row i holds visit column k = i + 1 + t;
+        // visit index v is block row v ^ xm
+        const int i = e >> 5, k = i + 1 + (e & 31);
+        const int ri = i ^ (int)xm, rk = k ^ (int)xm;
+        const bool ok = e >= 0 && e < kB * kB && k < kB && pos0 + ri < n && pos0 + rk < n;
+        jb_ok[0] |= (uint32_t)ok << e2;
+        jb_reg[0][e2] = p.Jt[ok ? (size_t)(pos0 + ri) * n + pos0 + rk : 0];
+      }
+      const bool hok = hl >= 0 && hl < kB && pos0 + hl < n;
+      jb_ok[0] |= (uint32_t)hok << 31;
+      h_reg[0] = p.h[hok ? pos0 + hl : 0];
+      if (!I8 && p.rowsum) rs_reg = p.rowsum[hok ? pos0 + hl : 0];
+    } else {  // each tile's instance (S symmetric: S^T = S; no row sums with int8)
+#pragma unroll
+      for (int ti = 0; ti < NI; ++ti) {
+        const int ni = tile(ti).n;
+        const int8_t* sg = p.sgn + tile(ti).sgn;
+        jb_ok[ti] = 0;
+#pragma unroll
+        for (int e2 = 0; e2 < NJB; ++e2) {
+          const int e = hl + e2 * NHL;
+          const int i = e >> 5, k = i + 1 + (e & 31);
+          const int ri = i ^ (int)xm, rk = k ^ (int)xm;
+          const bool ok = e >= 0 && e < kB * kB && k < kB && pos0 + ri < ni && pos0 + rk < ni;
+          jb_ok[ti] |= (uint32_t)ok << e2;
+          jb_reg[ti][e2] = (float)sg[ok ? (size_t)(pos0 + ri) * tile(ti).ld + pos0 + rk : 0];
+        }
+        const bool hok = hl >= 0 && hl < kB && pos0 + hl < ni;
+        jb_ok[ti] |= (uint32_t)hok << 31;
+        const int64_t ho = tile(ti).h;
+        h_reg[ti] = ho >= 0 ? p.h[ho + (hok ? pos0 + hl : 0)] : 0.f;
+      }
     }
-    const bool hok = hl >= 0 && hl < kB && pos0 + hl < n;
-    jb_ok |= (uint32_t)hok << 31;
-    h_reg = (BAT && !p.h) ? 0.f : p.h[hok ? pos0 + hl : 0];
-    if (!I8 && p.rowsum) rs_reg = p.rowsum[hok ? pos0 + hl : 0];
   };
   auto publish_block = [&](int pb) {
 #pragma unroll
-    for (int e2 = 0; e2 < NJB; ++e2) {
-      const int e = hl + e2 * NHL;
-      if (hl >= 0 && e < kB * kB) L.jb[pb][e] = ((jb_ok >> e2) & 1u) ? jb_reg[e2] : 0.f;
-    }
+    for (int ti = 0; ti < NI; ++ti)
+#pragma unroll
+      for (int e2 = 0; e2 < NJB; ++e2) {
+        const int e = hl + e2 * NHL;
+        if (hl >= 0 && e < kB * kB) L.jb[pb][ti][e] = ((jb_ok[ti] >> e2) & 1u) ? jb_reg[ti][e2] : 0.f;
+      }
     if (hl >= 0 && hl < kB) {
-      const bool hok = (jb_ok >> 31) & 1u;
-      L.hb[pb][hl] = hok ? h_reg : 0.f;
-      L.rsb[pb][hl] = (hok && !I8 && p.rowsum) ? rs_reg : 0.f;
+#pragma unroll
+      for (int ti = 0; ti < NI; ++ti) L.hb[pb][ti][hl] = ((jb_ok[ti] >> 31) & 1u) ? h_reg[ti] : 0.f;
+      L.rsb[pb][hl] = (((jb_ok[0] >> 31) & 1u) && !I8 && p.rowsum) ? rs_reg : 0.f;
     }
   };
   // Products of K step j (fragments fa) with this tile's spins hx.
@@ -324,9 +383,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         if (j < 2 * NW && u < jrem && j != jx) {
 #pragma unroll
           for (int tt = 0; tt < NT; ++tt)
-            mfma_step(ring[u], get_half<NW>(sa[tt], j), acc[tt], iacc[tt]);
+            mfma_step(ring[BAT ? tt : 0][u], get_half<NW>(sa[tt], j), acc[tt], iacc[tt]);
         }
-        load_stage(cr, j + PF, ring[u]);
+#pragma unroll
+        for (int ti = 0; ti < NI; ++ti) load_stage(cr, j + PF, ring[ti][u], ti);
       }
     }
 #pragma unroll
@@ -347,7 +407,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         accept_bounds_f(wd[e], cb, cz, lo[e], hi[e]);
-        if (pos0 + 4 * g4 + e >= n || !act) lo[e] = hi[e] = INFINITY;  // padding, idle lanes
+        if (pos0 + 4 * g4 + e >= nq || !act) lo[e] = hi[e] = INFINITY;  // padding, idle lanes
       }
       // rows 4 g4 + e are visits 4 (g4 ^ xm / 4) + (e ^ xm % 4): one 16-byte
       // store each, its entries exchanged by the uniform low bits of xm
@@ -471,7 +531,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   // rsb (zero otherwise); fsc is a power of two, so the fma rounds like the add
   const bool qsum = !I8 && p.ptype == 1;
   const float fsc = qsum ? 0.5f : 1.f;
-  const float bsc = BAT ? p.scale : 1.f;  // batched: field = bsc * count + h
+  // batched: field = bsc * count + h, the scale of the lane's instance
+  const float bsc = BAT ? (tq ? tile(1).scale : tile(0).scale) : 1.f;
   double eoff = 0.0;  // wave 0: caller's initial energy minus its recomputation
 
   // The correction J[c1, c] . S_c of the next block c1 by the proposal wave,
@@ -479,17 +540,20 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   // bits = block c's spins of replica `lane` in block-row order; the A
   // fragments of row block c1, K block c were loaded into fa at the block's
   // start.  Result to L.corr (read by this wave at the next block's start).
-  typedef Frag FragSet[NST][NP];
+  typedef Frag FragSet[NI][NST][NP];  // by instance
   auto load_corr_frags = [&](int c1, int c, FragSet& fa) {
-    const Frag* fb = reinterpret_cast<const Frag*>(p.frag);
 #pragma unroll
-    for (int st = 0; st < NST; ++st)
+    for (int ti = 0; ti < NI; ++ti) {
+      const Frag* fb = frag_base(ti);
 #pragma unroll
-      for (int pc = 0; pc < NP; ++pc) {
-        const size_t f = I8 ? (size_t)c1 * nblk + c
-                            : ((size_t)c1 * (2 * nblk) + 2 * c + st) * NP + pc;
-        fa[st][pc] = fb[f * 64 + lane];
-      }
+      for (int st = 0; st < NST; ++st)
+#pragma unroll
+        for (int pc = 0; pc < NP; ++pc) {
+          const size_t f = I8 ? (size_t)c1 * nblk + c
+                              : ((size_t)c1 * (2 * nblk) + 2 * c + st) * NP + pc;
+          fa[ti][st][pc] = fb[f * 64 + lane];
+        }
+    }
   };
   auto corr_products = [&](const FragSet& fa, uint32_t bits, int ql) {  // ql: lane
 #pragma unroll
@@ -498,7 +562,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
       const uint32_t bt = (uint32_t)__shfl((int)bits, 32 * tl + (ql & 31), 64);
       f32x16 a = {};
       i32x16 ia = {};
-      mfma_step(fa, half_of<I8>(bt, h), a, ia);
+      mfma_step(fa[BAT ? tl : 0], half_of<I8>(bt, h), a, ia);
       write_fields_col(L.corr, a, ia, 32 * tl + (ql & 31));
     }
   };
@@ -541,14 +605,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           // ---- dry pass: f64 energy of the block's rows from its full fields
           const uint32_t bits = L.bb[pb][qo];
           for (int i = 0; i < kB; ++i) {
-            if (pos0 + i < n) {
+            if (pos0 + i < nq) {
               const int o = fld(qo, i);
               double f = (double)L.part[pb][0][o];
               for (int mm = 1; mm < MW; ++mm) f += (double)L.part[pb][mm][o];
               f += (double)L.corr[o];
               if (qsum) f = 0.5 * (f + (double)L.rsb[pb][i]);
+              if (BAT) f *= (double)bsc;
               const double si = ((bits >> i) & 1u) ? 1.0 : (p.ptype == 0 ? -1.0 : 0.0);
-              eacc += si * (0.5 * f + (double)L.hb[pb][i]);
+              eacc += si * (0.5 * f + (double)L.hb[pb][tq][i]);
             }
           }
           L.bnew[pb][qo] = bits;
@@ -598,7 +663,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                 for (int mm = 0; mm < MW; ++mm)
                   ld[j][mm] = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
                 ld[j][MW] = *reinterpret_cast<const float4*>(&L.corr[o]);
-                ld[j][MW + 1] = *reinterpret_cast<const float4*>(&L.hb[pb][4 * gr]);
+                ld[j][MW + 1] = *reinterpret_cast<const float4*>(&L.hb[pb][tq][4 * gr]);
                 ld[j][MW + 2] = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
               }
               __builtin_amdgcn_sched_barrier(0);
@@ -620,8 +685,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                 for (int e = 0; e < 4; ++e) {
                   if constexpr (BAT) {
                     fp[4 * (g0 + j) + e] = vs[e ^ P];
-                    hk[4 * (g0 + j) + e] =
-                        __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(hv[e ^ P])));
+                    hk[4 * (g0 + j) + e] = hv[e ^ P];
                   } else {
                     fp[4 * (g0 + j) + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
                   }
@@ -655,7 +719,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         // columns right of the diagonal, shifted to the start of the row, so
         // the ceil((31 - k) / 4) 16-byte loads of a row carry no dead lanes.
         constexpr int JR = 2;
-        const float4* jb4 = reinterpret_cast<const float4*>(L.jb[pb]);
+        const float4* jb4 = reinterpret_cast<const float4*>(L.jb[pb][tq]);
         float4 jring[JR][kB / 4];
         static_for<JR>([&](auto dc) __attribute__((always_inline)) {
           constexpr int d = decltype(dc)::value;
@@ -720,7 +784,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             float v = L.part[pb][0][fld(qo, br)];
             for (int mm = 1; mm < MW; ++mm) v = v + L.part[pb][mm][fld(qo, br)];
             v = v + L.corr[fld(qo, br)];
-            sf[r] = BAT ? v : __builtin_fmaf(fsc, v + L.rsb[pb][br], L.hb[pb][br]);
+            sf[r] = BAT ? v : __builtin_fmaf(fsc, v + L.rsb[pb][br], L.hb[pb][tq][br]);
           }
           mb = mb0;
           dpart = 0.f;
@@ -728,7 +792,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           for (int k = 0; k < kB; ++k) {
             const bool up = (mb >> k) & 1u;
             const float sg = up ? cu : -cu;
-            const float fk = BAT ? sf[k] * bsc + L.hb[pb][k ^ (int)xm] : sf[k];
+            const float fk = BAT ? sf[k] * bsc + L.hb[pb][tq][k ^ (int)xm] : sf[k];
             const float dn = fk * sg;
             // idle lanes and padding rows have lo = hi = +inf: never taken
             bool take = act & (dn > L.hi[pb][qo][k]);
@@ -744,7 +808,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
               dpart += take ? -dn : 0.f;
             const float dS = take ? -sg : 0.f;
             for (int r = k + 1; r < kB; ++r)
-              sf[r] = __builtin_fmaf(L.jb[pb][k * kB + (r - k - 1)], dS, sf[r]);
+              sf[r] = __builtin_fmaf(L.jb[pb][tq][k * kB + (r - k - 1)], dS, sf[r]);
           }
         }
         {
@@ -777,7 +841,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                                         (uint32_t)rr & 3u));
           __syncthreads();
           if (act && rr == 0) {  // read leader
-            const int b0 = g * R;
+            const int b0 = BAT ? 32 * tq + gl * R : g * R;  // the read's first lane
             int src = -1;
             if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
               float cur = gE;
@@ -796,7 +860,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
               for (int k = 0; k < R - 1; ++k) {
                 const int a = R - k - 1, b = R - k - 2;
                 const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
-                                 (p.betas[a] - p.betas[b]);
+                                 (betas_q[a] - betas_q[b]);
                 if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
                   const int tt = L.bi[b0 + a];
                   L.bi[b0 + a] = L.bi[b0 + b];
@@ -804,12 +868,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                 }
               }
             }
-            for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = p.betas[k];
+            for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = betas_q[k];
             if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
               double lw = 0.0, bold = 0.0;
               for (int r = 1; r < R; ++r) {
                 const int k = p.bsort[r];
-                const float be = p.betas[k];
+                const float be = betas_q[k];
                 const float en = L.E[b0 + L.bi[b0 + k]];
                 const float a = -be * en;
                 lw += (double)a - (-bold * (double)en);
@@ -870,14 +934,14 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt)
               if (mw && p.best_sem == 0 && actt[tt] && L.flag[qtt[tt]])
-                store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype, p.best_rep + base_t[tt]);
+                store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype,
+                                    p.best_rep + base_t[tt]);
             __syncthreads();
             // owners copy their part of a new global best
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt)
-              if (mw && p.best_sem == 1 && actt[tt] && L.src[gtt[tt]] == qtt[tt] - gtt[tt] * R)
-                store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype,
-                                    p.best_state + (size_t)(best_adj + (int64_t)(read0 + gtt[tt]) * n));
+              if (mw && p.best_sem == 1 && actt[tt] && L.src[gtt[tt]] == rtt_of(tt))
+                store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype, best_of(tt));
           }
           // the next pass's first block (after the exchanges: new betas)
           if (hw && w > MW + 1 && !dry_next) brackets(c1, m1, sw_next, pn);
@@ -893,7 +957,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   // ---- outputs
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt)
-    if (actt[tt]) store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype, p.spins + base_t[tt]);
+    if (actt[tt]) store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype, p.spins + base_t[tt]);
   if (w == 0 && act) {
     if (p.accepted) atomicAdd(p.accepted + read, acc);
     p.energies[(size_t)read * R + rr] = E;
@@ -909,9 +973,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
     __syncthreads();
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt)
-      if (actt[tt] && qtt[tt] - gtt[tt] * R == 0 && !L.src[gtt[tt]])
-        store_range<NW, I8>(sa[tt], c0, kb, nblk, n, p.ptype,
-                            p.best_state + (size_t)(best_adj + (int64_t)(read0 + gtt[tt]) * n));
+      if (actt[tt] && rtt_of(tt) == 0 && !L.src[gtt[tt]])
+        store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype, best_of(tt));
   }
 }
 
@@ -929,14 +992,14 @@ hipError_t launch_pipe_o(const BParams& p, int nw, int groups, hipStream_t st) {
 
 // Launch of one coupling encoding: nw spin words per matrix wave, nt tiles of
 // 32 replicas per workgroup, the order from p.
-template <int NP, bool I8, bool BAT = false>
+template <int NP, bool I8>
 hipError_t launch_pipe_enc(const BParams& p, int nw, int nt, int groups, hipStream_t st) {
   if (nt != 1 && nt != 2) return hipErrorInvalidValue;
   if (p.order == kBlockRandom)
-    return nt == 2 ? launch_pipe_o<NP, I8, true, 2, BAT>(p, nw, groups, st)
-                   : launch_pipe_o<NP, I8, true, 1, BAT>(p, nw, groups, st);
-  return nt == 2 ? launch_pipe_o<NP, I8, false, 2, BAT>(p, nw, groups, st)
-                 : launch_pipe_o<NP, I8, false, 1, BAT>(p, nw, groups, st);
+    return nt == 2 ? launch_pipe_o<NP, I8, true, 2>(p, nw, groups, st)
+                   : launch_pipe_o<NP, I8, true, 1>(p, nw, groups, st);
+  return nt == 2 ? launch_pipe_o<NP, I8, false, 2>(p, nw, groups, st)
+                 : launch_pipe_o<NP, I8, false, 1>(p, nw, groups, st);
 }
 
 // One per translation unit (blocked_pipe_i8.hip, blocked_pipe_bf16_{1,2,3}.hip).
@@ -945,7 +1008,7 @@ hipError_t launch_pipe_bf16_1(const BParams& p, int nw, int nt, int groups, hipS
 hipError_t launch_pipe_bf16_2(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 hipError_t launch_pipe_bf16_3(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 // The batched form (blocked_pipe_bat.hip): one workgroup per entry of p.bat.
-hipError_t launch_pipe_bat(const BParams& p, int nw, int nt, int groups, hipStream_t st);
+hipError_t launch_pipe_bat(const BParams& p, int nw, int groups, hipStream_t st);
 
 }  // namespace blk
 }  // namespace pysa

@@ -1,6 +1,7 @@
 // Batched MFMA form: the pipelined kernel over many instances of different
-// sizes in one launch, one instance per workgroup (its entry of a BatWG
-// table), with the instances' int8 coupling matrices packed into MFMA
+// sizes in one launch, two tiles of up to 32 chains per workgroup (its entry
+// of a BatWG table; the tiles' instances have the same number of 32-spin
+// blocks), with the instances' int8 coupling matrices packed into MFMA
 // fragments at the start of the call.
 //
 // PySA has no batched path: a batch is a caller's loop over Solver objects,
@@ -14,8 +15,9 @@
 namespace pysa {
 namespace blk {
 
-hipError_t launch_pipe_bat(const BParams& p, int nw, int nt, int groups, hipStream_t st) {
-  return launch_pipe_enc<1, true, true>(p, nw, nt, groups, st);
+hipError_t launch_pipe_bat(const BParams& p, int nw, int groups, hipStream_t st) {
+  return p.order == kBlockRandom ? launch_pipe_o<1, true, true, 2, true>(p, nw, groups, st)
+                                 : launch_pipe_o<1, true, false, 2, true>(p, nw, groups, st);
 }
 
 namespace {
@@ -42,12 +44,13 @@ __global__ void pack_i8_batched_kernel(const int8_t* __restrict__ signs,
 }
 
 // Per-read minimum over the replicas' bests (simulation.py:174-177, first
-// minimum wins), one thread per read of one workgroup's entry.
+// minimum wins), one thread per read slot of a workgroup's tiles.
 __global__ void bat_best_argmin_kernel(BParams p, int groups) {
   const int gw = blockIdx.x * blockDim.x + threadIdx.x;
-  const int wg = gw / p.rpw, j = gw - wg * p.rpw;
+  const int per = 2 * p.rpw;
+  const int wg = gw / per, tt = (gw - wg * per) / p.rpw, j = gw - wg * per - tt * p.rpw;
   if (wg >= groups) return;
-  const BatWG b = p.bat[wg];
+  const BatTile& b = p.bat[wg].t[tt];
   const int read = b.read0 + j;
   if (read >= b.read_end) return;
   const int R = p.R, n = b.n;
@@ -77,7 +80,7 @@ hipError_t pack_batched_frags(const int8_t* signs, const BatPack* tab, int count
 }
 
 hipError_t bat_best_argmin(const BParams& p, int groups, hipStream_t st) {
-  const long long threads = (long long)groups * p.rpw;
+  const long long threads = (long long)groups * 2 * p.rpw;
   hipLaunchKernelGGL(bat_best_argmin_kernel, dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st,
                      p, groups);
   return hipGetLastError();
@@ -98,13 +101,15 @@ hipError_t BatchedMfmaRun::enqueue(const BatchParams& bp, const std::vector<int>
   if (inst.empty()) return hipSuccess;
   const int R = bp.R, reads = bp.reads;
   if (R < 1 || R > 32) return hipErrorInvalidValue;
-  // tiles of 32 chains per workgroup: two when an instance has more than 32
-  const int nt = (int64_t)reads * R > 32 ? 2 : 1;
-  const int rpw = 32 * nt / R;
+  const int rpt = 32 / R;  // reads per tile of 32 chains
+  // Tiles (up to rpt reads of one instance), largest instances first; two
+  // consecutive tiles of the same block count share a workgroup.
   int64_t foff = 0;
   int max_nblk = 0;
   pk.clear();
   wg.clear();
+  std::vector<BatTile> tiles;
+  std::vector<int> tnb;
   for (int k : inst) {
     const int n = hn[k], nblk = (n + kB - 1) / kB;
     if (n < kBatchedMfmaMinN || n > kBatchedMaxN) return hipErrorInvalidValue;
@@ -117,24 +122,35 @@ hipError_t BatchedMfmaRun::enqueue(const BatchParams& bp, const std::vector<int>
     b.frag = foff;
     pk.push_back(b);
     max_nblk = nblk > max_nblk ? nblk : max_nblk;
-    for (int r0 = 0; r0 < reads; r0 += rpw) {
-      BatWG g;
-      g.n = n;
-      g.nblk = nblk;
-      g.kb = (nblk + kPipeMW - 1) / kPipeMW;
-      g.ld = hld[k];
-      g.inst = k;
-      g.read0 = k * reads + r0;
-      g.read_end = (k + 1) * reads;
-      g.scale = hscale[k];
-      g.frag = foff;
-      g.sgn = hj[k];
-      g.h = hh_off ? hh_off[k] : -1;
-      g.spin_adj = hs[k] - (int64_t)k * reads * R * n;
-      g.best_adj = hs[k] / R - (int64_t)k * reads * n;
-      wg.push_back(g);
+    for (int r0 = 0; r0 < reads; r0 += rpt) {
+      BatTile t;
+      t.n = n;
+      t.ld = hld[k];
+      t.inst = k;
+      t.read0 = k * reads + r0;
+      t.read_end = k * reads + (r0 + rpt < reads ? r0 + rpt : reads);
+      t.scale = hscale[k];
+      t.frag = foff;
+      t.sgn = hj[k];
+      t.h = hh_off ? hh_off[k] : -1;
+      t.spin_adj = hs[k] - (int64_t)k * reads * R * n;
+      t.best_adj = hs[k] / R - (int64_t)k * reads * n;
+      tiles.push_back(t);
+      tnb.push_back(nblk);
     }
     foff += (int64_t)nblk * nblk * 64 * 16;
+  }
+  for (size_t i = 0; i < tiles.size();) {
+    BatWG g;
+    g.nblk = tnb[i];
+    g.kb = (g.nblk + kPipeMW - 1) / kPipeMW;
+    g.pad0 = g.pad1 = 0;
+    g.t[0] = tiles[i];
+    const bool pair = i + 1 < tiles.size() && tnb[i + 1] == tnb[i];
+    g.t[1] = pair ? tiles[i + 1] : tiles[i];
+    if (!pair) g.t[1].read_end = g.t[1].read0;  // an empty tile (valid pointers, no reads)
+    wg.push_back(g);
+    i += pair ? 2 : 1;
   }
   const int kbmax = (max_nblk + kPipeMW - 1) / kPipeMW;
   int nw = 4;
@@ -157,7 +173,7 @@ hipError_t BatchedMfmaRun::enqueue(const BatchParams& bp, const std::vector<int>
   p.R = R;
   p.reads = bp.count * reads;
   p.S = bp.S;
-  p.rpw = rpw;
+  p.rpw = rpt;  // reads per tile
   p.np = 1;
   p.use_pt = bp.use_pt;
   p.logw = 0;
@@ -183,7 +199,7 @@ hipError_t BatchedMfmaRun::enqueue(const BatchParams& bp, const std::vector<int>
   p.accepted = bp.accepted;
   p.bat = (const BatWG*)tab;
   p.sgn = bp.signs;
-  e = launch_pipe_bat(p, nw, nt, (int)wg.size(), st);
+  e = launch_pipe_bat(p, nw, (int)wg.size(), st);
   if (e == hipSuccess && bp.best_sem == 0) e = bat_best_argmin(p, (int)wg.size(), st);
   return e;
 }

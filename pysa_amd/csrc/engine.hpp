@@ -116,21 +116,26 @@ constexpr int kBatchedMaxN = 2048;
 hipError_t launch_batched(const BatchParams& p, int items, int max_n, hipStream_t st);
 
 // ---- batched MFMA form (blocked_pipe_bat.hip) ----
-// One workgroup of the batched MFMA form: the instance it runs and where
-// that instance's data lie.  States of global read g, replica r start at
-// spin_adj + (g R + r) n (spin_adj = the instance's state offset minus the
-// global index of its first chain times n), its best state at
-// best_adj + g n; the offsets are never negative once a chain's own term is
-// added.
-struct BatWG {
-  int n, nblk, kb, ld;  // spins, 32-spin blocks, K blocks per matrix wave, row stride of S
-  int inst, read0, read_end;  // instance; first global read of the group; end of its reads
+// The batched MFMA form runs two tiles of up to 32 chains per workgroup,
+// each tile some reads of one instance; the two instances have the same
+// number of 32-spin blocks, so they share the block schedule.  States of
+// global read g, replica r of a tile start at spin_adj + (g R + r) n
+// (spin_adj = the instance's state offset minus the global index of its
+// first chain times n), its best state at best_adj + g n; the offsets are
+// never negative once a chain's own term is added.  An empty tile has
+// read0 == read_end.
+struct BatTile {
+  int n, ld, inst, read0, read_end;  // spins, row stride of S, instance, global reads [read0, read_end)
   float scale;
   int64_t frag;      // byte offset of the instance's int8 MFMA fragments
   int64_t sgn;       // offset of its int8 coupling matrix S
   int64_t h;         // offset of its local fields, -1 when h = 0
   int64_t spin_adj;  // see above
   int64_t best_adj;
+};
+struct BatWG {
+  int nblk, kb, pad0, pad1;  // 32-spin blocks, K blocks per matrix wave
+  BatTile t[2];
 };
 // One instance of the fragment packing.
 struct BatPack {

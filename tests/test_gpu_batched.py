@@ -124,6 +124,17 @@ def test_batched_mfma_tiles(reads, R):
     compare(got, want, pk, reads, R)
 
 
+@pytest.mark.parametrize("ptype", [0, 1])
+def test_batched_mfma_paired_instances(ptype):
+    """Workgroups pair tiles of different instances with the same number of
+    32-spin blocks (sizes within one block range differ)."""
+    rng = np.random.default_rng(650 + ptype)
+    inst = make_instances(rng, [65, 70, 80, 96, 97, 100, 127, 128, 700, 704, 768], "sk") + \
+        make_instances(rng, [90, 120], "int")
+    got, want, pk = run_both(inst, ptype, 4, 8, 30, 3, ptype, 13, True, rng, form="blocked")
+    compare(got, want, pk, 4, 8)
+
+
 def test_batched_mfma_large_instances():
     """Up to 2048 spins: 16 spin words per matrix wave."""
     rng = np.random.default_rng(700)
