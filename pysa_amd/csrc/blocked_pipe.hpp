@@ -65,6 +65,8 @@ struct __align__(16) PipeLds {
   uint8_t msk[2][64];
   float E[64];
   float rb[64];
+  float bet[64];  // the ladder by slot (batched: tile t's instance's at 32 t)
+  int bso[64];    // its ascending order (log-omega)
   int bi[64];
   int flag[64];
   int src[64];
@@ -191,6 +193,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
     E = act ? p.energies[(size_t)read * R + rr] : 0.f;
     L.bi[q] = act ? rr : 0;  // beta_idx = arange(R) (sa.py:325)
     L.rb[q] = act ? betas_q[rr] : 0.f;
+    // the ladder in LDS for the exchanges (read serially by the read leaders)
+    const int kq = BAT ? (q & 31) : q;
+    L.bet[q] = kq < R ? betas_q[kq] : 0.f;
+    L.bso[q] = (kq < R && p.logw && p.bsort) ? p.bsort[kq] : 0;
     bestE = E;
     gE = E;  // simulation.py:205 (meaningful on read leaders, rr == 0)
     if (p.best_sem == 0 && act) {
@@ -836,12 +842,10 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             }
             L.flag[q] = imp ? 1 : 0;
           }
-          if (p.use_pt && act && rr < R - 1)
-            L.lu[q] = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)rr, p.k0, p.k1),
-                                        (uint32_t)rr & 3u));
           __syncthreads();
           if (act && rr == 0) {  // read leader
             const int b0 = BAT ? 32 * tq + gl * R : g * R;  // the read's first lane
+            const float* bet = &L.bet[BAT ? 32 * tq : 0];
             int src = -1;
             if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
               float cur = gE;
@@ -860,7 +864,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
               for (int k = 0; k < R - 1; ++k) {
                 const int a = R - k - 1, b = R - k - 2;
                 const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
-                                 (betas_q[a] - betas_q[b]);
+                                 (bet[a] - bet[b]);
                 if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
                   const int tt = L.bi[b0 + a];
                   L.bi[b0 + a] = L.bi[b0 + b];
@@ -868,12 +872,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                 }
               }
             }
-            for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = betas_q[k];
+            for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = bet[k];
             if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
               double lw = 0.0, bold = 0.0;
               for (int r = 1; r < R; ++r) {
-                const int k = p.bsort[r];
-                const float be = betas_q[k];
+                const int k = L.bso[r];
+                const float be = bet[k];
                 const float en = L.E[b0 + L.bi[b0 + k]];
                 const float a = -be * en;
                 lw += (double)a - (-bold * (double)en);
@@ -920,6 +924,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           publish_block(pn);
           if (w > MW + 1 && !last && !dry) brackets(c1, m1, sw, pn);
           prefetch_jb(c2, m2);
+          // the sweep's exchange uniforms (utils.py:57), by a helper that has
+          // no brackets to compute during the last block
+          if (w == MW + 2 && last && !dry && p.use_pt && act && rr < R - 1)
+            L.lu[q] = log_u32(pick_word(pt_block((uint64_t)read, sw, (uint32_t)rr, p.k0, p.k1),
+                                        (uint32_t)rr & 3u));
         }
         __syncthreads();  // B
         // ---- the owners of block c take its new spins
