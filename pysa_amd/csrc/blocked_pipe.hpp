@@ -66,6 +66,7 @@ struct __align__(16) PipeLds {
   float E[64];
   float rb[64];
   float bet[64];  // the ladder by slot (batched: tile t's instance's at 32 t)
+  float Es[64];   // energy of the replica in each slot, before the exchanges
   int bso[64];    // its ascending order (log-omega)
   int bi[64];
   int flag[64];
@@ -843,36 +844,69 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             L.flag[q] = imp ? 1 : 0;
           }
           __syncthreads();
+          // lane b0 + k: the energy of the replica in slot k (the walk below
+          // reads the slots in order; the replicas' energies do not change).
+          // The lane offsets are derived here from an opaque copy of the lane
+          // index: hoisted to the kernel start they were kept in scratch.
+          int qx = q;
+          asm volatile("" : "+v"(qx));
+          const int tx = BAT ? (qx >> 5) : 0;
+          const int b0 = BAT ? 32 * tx + ((qx & 31) / R) * R : (qx / R) * R;  // the read's first lane
+          const int slot = qx - b0;
+          const float* bet = &L.bet[32 * tx];
+          if (act) L.Es[qx] = L.E[b0 + L.bi[qx]];
           if (act && rr == 0) {  // read leader
-            const int b0 = BAT ? 32 * tq + gl * R : g * R;  // the read's first lane
-            const float* bet = &L.bet[BAT ? 32 * tq : 0];
             int src = -1;
             if (p.best_sem == 1) {  // simulation.py:222-225 in replica order
               float cur = gE;
-              for (int k = 0; k < R; ++k)
-                if (L.E[b0 + k] < cur) {
-                  cur = L.E[b0 + k];
+              float en = L.E[b0];
+              for (int k = 0; k < R; ++k) {
+                const float ek = en;
+                if (k + 1 < R) en = L.E[b0 + k + 1];  // the next one, read ahead
+                if (ek < cur) {
+                  cur = ek;
                   src = k;
                 }
+              }
               if (src >= 0) {
                 gE = cur;
                 gsw = s;
                 gimp = 1;
               }
             }
-            if (p.use_pt) {  // utils.py:44-58
+            if (p.use_pt && R > 1) {  // utils.py:44-58
+              // Hot end first: slot a = R-k-1 holds the carried replica (the one
+              // the previous pair left there), slot b = a-1 still its occupant
+              // from before the walk.  An exchange moves the carried replica on
+              // to b; otherwise b's occupant becomes the carried one.  Every
+              // operand but the carried one is read one step ahead.
+              int carry = L.bi[b0 + R - 1];
+              float ec = L.Es[b0 + R - 1], ba = bet[R - 1];
+              int rbn = L.bi[b0 + R - 2];
+              float ebn = L.Es[b0 + R - 2], bbn = bet[R - 2];
+              double lun = L.lu[b0];
               for (int k = 0; k < R - 1; ++k) {
-                const int a = R - k - 1, b = R - k - 2;
-                const float dd = (L.E[b0 + L.bi[b0 + a]] - L.E[b0 + L.bi[b0 + b]]) *
-                                 (bet[a] - bet[b]);
-                if (dd >= 0.f || L.lu[b0 + k] < (double)dd) {
-                  const int tt = L.bi[b0 + a];
-                  L.bi[b0 + a] = L.bi[b0 + b];
-                  L.bi[b0 + b] = tt;
+                const int a = R - k - 1, b = a - 1;
+                const int rb = rbn;
+                const float eb = ebn, bb = bbn;
+                const double luk = lun;
+                if (b > 0) {
+                  rbn = L.bi[b0 + b - 1];
+                  ebn = L.Es[b0 + b - 1];
+                  bbn = bet[b - 1];
+                  lun = L.lu[b0 + k + 1];
                 }
+                const float dd = (ec - eb) * (ba - bb);
+                const bool x = dd >= 0.f || luk < (double)dd;
+                L.bi[b0 + a] = x ? rb : carry;
+                if (!x) {
+                  carry = rb;
+                  ec = eb;
+                }
+                ba = bb;
               }
+              L.bi[b0] = carry;
             }
-            for (int k = 0; k < R; ++k) L.rb[b0 + L.bi[b0 + k]] = bet[k];
             if (p.logw && p.log_omegas) {  // ais.py:99-112 over the ascending ladder
               double lw = 0.0, bold = 0.0;
               for (int r = 1; r < R; ++r) {
@@ -887,6 +921,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             }
             L.src[g] = src;
           }
+          // every slot's new beta to the replica now in it (after the
+          // leader's exchanges: the same wave, in order)
+          if (act) L.rb[b0 + L.bi[qx]] = bet[slot];
           __syncthreads();
           __syncthreads();  // the next pass's first brackets
         }
