@@ -180,13 +180,21 @@ int pysa_amd_metropolis_device_noalloc(int64_t handle, int32_t reads, int32_t R,
  *   states       in/out chains*n spins
  *   log_w        out chains log-weights (f64, without log Z0)
  *   energies     out chains final energies as tracked (f64: recomputed with
- *                the fields every 256 steps for float32 couplings, updated by
- *                the accepted changes in between), or NULL
+ *                the fields every 256 steps (plain) or 64 steps (MFMA form) for
+ *                float32 couplings, updated by the accepted changes in
+ *                between), or NULL
+ *   form         PYSA_AMD_FORM_PLAIN: the plain AIS kernel (every case);
+ *                BLOCKED / BLOCKED_EXACT: the MFMA form (2 or 3 bf16 pieces,
+ *                int8 for small integer couplings) for float32, the uniform
+ *                reference, no replay, order 0 or 3 and 65 <= n <= 2048
+ *                (PYSA_AMD_ENOTSUP otherwise); AUTO: BLOCKED where it applies,
+ *                else PLAIN.  With exactly encoded couplings every form gives
+ *                the same states and weights for the same order.
  */
 int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, int32_t reference,
                  double p, int32_t order, int32_t ordering, uint64_t seed,
                  const double* replay_log_r, const int32_t* replay_perm, int8_t* states,
-                 double* log_w, double* energies, double* kernel_ms);
+                 double* log_w, double* energies, int32_t form, double* kernel_ms);
 
 /*
  * Device-memory helpers for callers without their own HIP allocator (the

@@ -41,7 +41,7 @@ _SIGNATURES = {
                                            _c_i32, _c_i32, _vp, _vp, _c_u64, _c_u64, _c_i32,
                                            _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pysa_amd_ais": [_c_i64, _vp, _c_i32, _c_i32, _c_i32, _c_dbl, _c_i32, _c_i32, _c_u64,
-                     _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_c_dbl)],
+                     _vp, _vp, _vp, _vp, _vp, _c_i32, ctypes.POINTER(_c_dbl)],
     "pysa_amd_energy": [_c_i64, _vp, _c_i64, _vp],
     "pysa_amd_plain_fields": [_c_i64, _c_i64, _vp],
     "pysa_amd_problem_set_shape": [_c_i64, _c_i32, _c_i32],

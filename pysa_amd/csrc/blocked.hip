@@ -998,6 +998,94 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
   return 0;
 }
 
+int run_blocked_ais(BlockedProblem& b, const void* dJ, const void* dJt, const void* dh, int n,
+                    int ptype, int chains, int nt, const float* temps, int order, int ordering,
+                    int pieces, uint64_t seed, int8_t* states, double* energy, double* log_w,
+                    hipStream_t st, double* ms, std::string* why) {
+  if (!blocked_eligible(n, 0, 1, chains, order, false) || n <= 2 * kB) {
+    *why = "MFMA AIS needs float32 couplings, 65 <= n <= 2048, sequential or block-random order, "
+           "the uniform reference and Philox randomness";
+    return PYSA_AMD_ENOTSUP;
+  }
+  if (b.exact_kind < 0) {
+    std::vector<float> hJ((size_t)n * n), hh(n);
+    if (hipMemcpy(hJ.data(), dJ, hJ.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hh.data(), dh, hh.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      *why = "reading couplings back";
+      return PYSA_AMD_EHIP;
+    }
+    b.exact_kind = blocked_exact_pieces(hJ.data(), hJ.size());
+    b.exact_sums = integer_sums_exact(hJ.data(), hh.data(), n) ? 1 : 0;
+  }
+  // one chain per read (R = 1: the chain index is the oracle's), two tiles
+  // of 32 chains per workgroup on the pipelined kernel
+  BlockedPlan pl = plan_blocked(b, n, chains, 1, order, pieces);
+  pl.pipe = true;
+  pl.nt = 2;
+  pl.groups = (chains + 63) / 64;
+  int rc = prepare_encoding(b, dJ, dJt, n, ptype, pl, st, why);
+  if (rc) return rc;
+  BParams p{};
+  p.n = n;
+  p.nblk = pl.nblk;
+  p.ptype = ptype;
+  p.R = 1;
+  p.reads = chains;
+  p.S = nt;
+  p.rpw = 64;
+  p.np = pl.np;
+  p.use_pt = 0;
+  p.logw = 0;
+  p.best_sem = 0;
+  p.order = order;
+  // dry passes recompute the energy every 64 steps unless every sum is exact
+  const char* ev = getenv("PYSA_AMD_ENERGY_REFRESH");
+  p.refresh = ev && ev[0] ? atoi(ev) : (b.exact_sums == 1 ? 0 : 64);
+  p.kb = pl.kbp;
+  p.k0 = (uint32_t)seed;
+  p.k1 = (uint32_t)(seed >> 32);
+  p.sweep_offset = 0;
+  p.frag = b.packed;
+  p.Jt = pl.inexact ? b.jtl : (const float*)dJt;
+  p.h = (const float*)dh;
+  p.rowsum = (!pl.i8 && ptype == 1) ? b.rowsum : nullptr;
+  p.spins = states;
+  p.temps = temps;
+  p.ordering = ordering;
+  p.ais_e = energy;
+  p.ais_w = log_w;
+  EventPair ev2;
+  if (ev2.create() != hipSuccess) {
+    *why = "creating events";
+    return PYSA_AMD_EHIP;
+  }
+  (void)hipEventRecord(ev2.a, st);
+  hipError_t le;
+  if (pl.i8)
+    le = launch_pipe_ais_i8(p, pl.nwp, pl.groups, st);
+  else if (pl.np == 1)
+    le = launch_pipe_ais_bf16_1(p, pl.nwp, pl.groups, st);
+  else if (pl.np == 2)
+    le = launch_pipe_ais_bf16_2(p, pl.nwp, pl.groups, st);
+  else
+    le = launch_pipe_ais_bf16_3(p, pl.nwp, pl.groups, st);
+  (void)hipEventRecord(ev2.b, st);
+  if (le != hipSuccess) {
+    *why = std::string("MFMA AIS launch: ") + hipGetErrorString(le);
+    (void)hipStreamSynchronize(st);
+    return PYSA_AMD_EHIP;
+  }
+  if (hipEventSynchronize(ev2.b) != hipSuccess) {
+    *why = "MFMA AIS execution";
+    return PYSA_AMD_EHIP;
+  }
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, ev2.a, ev2.b);
+  if (ms) *ms = t;
+  b.last_kind = pl.kind;
+  return 0;
+}
+
 int run_blocked_noalloc(const BlockedProblem& b, const void* dJt, const void* dh, int n,
                         int ptype, int dtype, int reads, int R, int S, int order, int use_pt,
                         int logw, int best_sem, int pieces, uint64_t seed, uint64_t sweep_offset,

@@ -69,4 +69,13 @@ int run_blocked_noalloc(const BlockedProblem& b, const void* dJt, const void* dh
 // device (looked up on the first call per device).
 int noalloc_scratch(void** ptr, size_t* bytes, std::string* why);
 
+// AIS with the uniform reference on the pipelined MFMA kernel (float32,
+// 65 <= n <= 2048, sequential or block-random order), all arrays in device
+// memory: `chains` chains of n spins, the schedule temps[nt], energy in (the
+// f64 energies of the initial states) and out (tracked), log_w out.
+int run_blocked_ais(BlockedProblem& b, const void* dJ, const void* dJt, const void* dh, int n,
+                    int ptype, int chains, int nt, const float* temps, int order, int ordering,
+                    int pieces, uint64_t seed, int8_t* states, double* energy, double* log_w,
+                    hipStream_t st, double* ms, std::string* why);
+
 }  // namespace pysa

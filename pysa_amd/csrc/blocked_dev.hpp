@@ -159,6 +159,14 @@ struct BParams {
   // betas and frag are then the batch's base pointers
   const BatWG* bat;
   const int8_t* sgn;
+  // AIS (pipelined kernel with kModeAIS): the schedule temps[S] (the beta of
+  // each sweep), the weight ordering (0: after the sweep, 1: before it), and
+  // per chain (read g, R = 1) the f64 energy (in: initial, out: tracked) and
+  // the log-weight (out)
+  const float* temps;
+  int ordering;
+  double* ais_e;
+  double* ais_w;
 };
 
 // Store the spins of one operand-role replica for the blocks this wave owns.

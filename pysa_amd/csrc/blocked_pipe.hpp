@@ -24,6 +24,8 @@ namespace blk {
 // running block counter.
 // K ranges of the pipelined kernel (one matrix wave each).
 constexpr int kPipeMW = 3;
+// What the pipelined kernel runs (its MODE).
+constexpr int kModePT = 0, kModeBatched = 1, kModeAIS = 2;
 constexpr int kPipePF = 3;  // fragment stages in flight per matrix wave (slot ring)
 
 // A replica's 32 fields take 32 floats; its 16-byte row groups are XOR-
@@ -83,8 +85,14 @@ struct __align__(16) PipeLds {
 // fields kept as the exact integer counts S s, and the field of a spin is the
 // f32 value scale * c + h, rounded twice like the batched chain of the
 // oracle; the energy changes are summed in f64 in visit order.
-template <int NP, bool I8, int NW, int MW, bool BR, int NT, bool BAT = false>
+// MODE: kModePT (Metropolis sweeps with parallel tempering), kModeBatched
+// (BAT), kModeAIS (AIS: annealed importance sampling with the uniform
+// reference; no exchanges, no best tracking; every chain at the schedule's
+// beta of the sweep, its energy tracked in f64 and its log-weight updated
+// after every sweep, pysa/simulation.py:293-317).
+template <int NP, bool I8, int NW, int MW, bool BR, int NT, int MODE = kModePT>
 __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0) {
+  constexpr bool BAT = MODE == kModeBatched, AIS = MODE == kModeAIS;
   // a local copy (the register allocation of the block loops was found to be
   // sensitive to this; the copy keeps it as measured)
   BParams p = p0;
@@ -190,7 +198,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   float E = 0.f, bestE = 0.f, gE = 0.f;
   int gsw = 0, gimp = 0;
   unsigned long long acc = 0;
-  if (w == 0) {
+  double Ed = 0.0, lw = 0.0, bold = 0.0;  // AIS: f64 energy, log-weight, previous beta
+  if (AIS && w == 0) {
+    Ed = act ? p.ais_e[read] : 0.0;  // R = 1: chain = read
+    L.bi[q] = 0;
+    const float t0 = p.temps[0];
+    L.rb[q] = act ? t0 : 0.f;
+    // ordering 1: the weight of sweep 0 before it (bold = 0)
+    if (p.ordering == 1 && act) lw = lw + ((-(double)t0 * Ed) - (-bold * Ed));
+  }
+  if (!AIS && w == 0) {
     E = act ? p.energies[(size_t)read * R + rr] : 0.f;
     L.bi[q] = act ? rr : 0;  // beta_idx = arange(R) (sa.py:325)
     L.rb[q] = act ? betas_q[rr] : 0.f;
@@ -629,10 +646,17 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           __syncthreads();  // B
           if (lastb) {
             if (act) {
-              if (ps == 0)
-                eoff = (double)E - eacc;
-              else
-                E = (float)(eacc + eoff);
+              if constexpr (AIS) {
+                if (ps == 0)
+                  eoff = Ed - eacc;
+                else
+                  Ed = eacc + eoff;
+              } else {
+                if (ps == 0)
+                  eoff = (double)E - eacc;
+                else
+                  E = (float)(eacc + eoff);
+              }
             }
             __syncthreads();  // the next pass's first brackets
           }
@@ -829,7 +853,24 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         if constexpr (!BAT) dsum += (double)dpart;
         nacc += (uint32_t)__builtin_popcount(mb ^ mb0);
         __syncthreads();  // B
-        if (lastb) {
+        if (AIS && lastb) {
+          // ---- end of an AIS step (simulation.py:305-316 with the uniform
+          // reference: its log-density is 0): the energy after the sweep, the
+          // weight of this step (ordering 0) or of the next one before its
+          // sweep (ordering 1), and the next step's beta for every chain
+          acc += nacc;
+          Ed = Ed + dsum;
+          const double bt = (double)p.temps[s];
+          if (act && p.ordering == 0) lw = lw + ((-bt * Ed) - (-bold * Ed));
+          bold = bt;
+          const bool more = s + 1 < p.S;
+          const float tn = more ? p.temps[s + 1] : 0.f;
+          if (act && p.ordering == 1 && more) lw = lw + ((-(double)tn * Ed) - (-bold * Ed));
+          L.rb[q] = act ? tn : 0.f;
+          __syncthreads();
+          __syncthreads();
+          __syncthreads();  // the next pass's first brackets
+        } else if (lastb) {
           // ---- end of sweep: energy (simulation.py:157-158), best states, PT
           acc += nacc;
           E = (float)((double)E + dsum);
@@ -979,14 +1020,14 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             __syncthreads();
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt)
-              if (mw && p.best_sem == 0 && actt[tt] && L.flag[qtt[tt]])
+              if (!AIS && mw && p.best_sem == 0 && actt[tt] && L.flag[qtt[tt]])
                 store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype,
                                     p.best_rep + base_t[tt]);
             __syncthreads();
             // owners copy their part of a new global best
 #pragma unroll
             for (int tt = 0; tt < NT; ++tt)
-              if (mw && p.best_sem == 1 && actt[tt] && L.src[gtt[tt]] == rtt_of(tt))
+              if (!AIS && mw && p.best_sem == 1 && actt[tt] && L.src[gtt[tt]] == rtt_of(tt))
                 store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype, best_of(tt));
           }
           // the next pass's first block (after the exchanges: new betas)
@@ -1004,7 +1045,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
 #pragma unroll
   for (int tt = 0; tt < NT; ++tt)
     if (actt[tt]) store_range<NW, I8>(sa[tt], c0, kb, nblk, n_of(tt), p.ptype, p.spins + base_t[tt]);
-  if (w == 0 && act) {
+  if (AIS && w == 0 && act) {
+    if (p.accepted) atomicAdd(p.accepted + read, acc);
+    p.ais_e[read] = Ed;
+    p.ais_w[read] = lw;
+  }
+  if (!AIS && w == 0 && act) {
     if (p.accepted) atomicAdd(p.accepted + read, acc);
     p.energies[(size_t)read * R + rr] = E;
     p.beta_idx[(size_t)read * R + rr] = L.bi[q];
@@ -1014,7 +1060,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
       L.src[g] = gimp;
     }
   }
-  if (p.best_sem == 1) {
+  if (!AIS && p.best_sem == 1) {
     // never improved: best_state is still the view of states[0] (simulation.py:204)
     __syncthreads();
 #pragma unroll
@@ -1024,13 +1070,13 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   }
 }
 
-template <int NP, bool I8, bool BR, int NT, bool BAT = false>
+template <int NP, bool I8, bool BR, int NT, int MODE = kModePT>
 hipError_t launch_pipe_o(const BParams& p, int nw, int groups, hipStream_t st) {
   dim3 grid(groups), block(128 * (kPipeMW + 1));
   switch (nw) {
-    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW, BR, NT, BAT>), grid, block, 0, st, p); break;
-    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW, BR, NT, BAT>), grid, block, 0, st, p); break;
-    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW, BR, NT, BAT>), grid, block, 0, st, p); break;
+    case 4: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 4, kPipeMW, BR, NT, MODE>), grid, block, 0, st, p); break;
+    case 8: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 8, kPipeMW, BR, NT, MODE>), grid, block, 0, st, p); break;
+    case 16: hipLaunchKernelGGL((blocked_pipe_kernel<NP, I8, 16, kPipeMW, BR, NT, MODE>), grid, block, 0, st, p); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1048,11 +1094,23 @@ hipError_t launch_pipe_enc(const BParams& p, int nw, int nt, int groups, hipStre
                  : launch_pipe_o<NP, I8, false, 1>(p, nw, groups, st);
 }
 
+// AIS (kModeAIS) with one coupling encoding: two tiles of 32 chains per
+// workgroup, the order from p.
+template <int NP, bool I8>
+hipError_t launch_pipe_ais_enc(const BParams& p, int nw, int groups, hipStream_t st) {
+  return p.order == kBlockRandom ? launch_pipe_o<NP, I8, true, 2, kModeAIS>(p, nw, groups, st)
+                                 : launch_pipe_o<NP, I8, false, 2, kModeAIS>(p, nw, groups, st);
+}
+
 // One per translation unit (blocked_pipe_i8.hip, blocked_pipe_bf16_{1,2,3}.hip).
 hipError_t launch_pipe_i8(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 hipError_t launch_pipe_bf16_1(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 hipError_t launch_pipe_bf16_2(const BParams& p, int nw, int nt, int groups, hipStream_t st);
 hipError_t launch_pipe_bf16_3(const BParams& p, int nw, int nt, int groups, hipStream_t st);
+hipError_t launch_pipe_ais_i8(const BParams& p, int nw, int groups, hipStream_t st);
+hipError_t launch_pipe_ais_bf16_1(const BParams& p, int nw, int groups, hipStream_t st);
+hipError_t launch_pipe_ais_bf16_2(const BParams& p, int nw, int groups, hipStream_t st);
+hipError_t launch_pipe_ais_bf16_3(const BParams& p, int nw, int groups, hipStream_t st);
 // The batched form (blocked_pipe_bat.hip): one workgroup per entry of p.bat.
 hipError_t launch_pipe_bat(const BParams& p, int nw, int groups, hipStream_t st);
 

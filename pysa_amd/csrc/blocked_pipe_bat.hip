@@ -16,8 +16,8 @@ namespace pysa {
 namespace blk {
 
 hipError_t launch_pipe_bat(const BParams& p, int nw, int groups, hipStream_t st) {
-  return p.order == kBlockRandom ? launch_pipe_o<1, true, true, 2, true>(p, nw, groups, st)
-                                 : launch_pipe_o<1, true, false, 2, true>(p, nw, groups, st);
+  return p.order == kBlockRandom ? launch_pipe_o<1, true, true, 2, kModeBatched>(p, nw, groups, st)
+                                 : launch_pipe_o<1, true, false, 2, kModeBatched>(p, nw, groups, st);
 }
 
 namespace {

@@ -8,5 +8,9 @@ hipError_t launch_pipe_bf16_1(const BParams& p, int nw, int nt, int groups, hipS
   return launch_pipe_enc<1, false>(p, nw, nt, groups, st);
 }
 
+hipError_t launch_pipe_ais_bf16_1(const BParams& p, int nw, int groups, hipStream_t st) {
+  return launch_pipe_ais_enc<1, false>(p, nw, groups, st);
+}
+
 }  // namespace blk
 }  // namespace pysa

@@ -615,7 +615,7 @@ int pysa_amd_metropolis(int64_t handle, int32_t reads, int32_t R, int32_t S, int
 int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, int32_t reference,
                  double p, int32_t order, int32_t ordering, uint64_t seed,
                  const double* replay_log_r, const int32_t* replay_perm, int8_t* states,
-                 double* log_w, double* energies, double* kernel_ms) {
+                 double* log_w, double* energies, int32_t form, double* kernel_ms) {
   g_err.clear();
   Problem* P = get(handle);
   if (!P) return fail(PYSA_AMD_EINVAL, "invalid problem handle");
@@ -624,6 +624,50 @@ int pysa_amd_ais(int64_t handle, const void* temps, int32_t nt, int32_t chains, 
   if (reference != 0 && reference != 1) return fail(PYSA_AMD_EINVAL, "reference must be 0 or 1");
   if (reference == 1 && !(p > 0.0 && p < 1.0)) return fail(PYSA_AMD_EINVAL, "Bernoulli p must be in (0, 1)");
   if (order < 0 || order > 3) return fail(PYSA_AMD_EINVAL, "order must be 0, 1, 2 or 3");
+  if (form != PYSA_AMD_FORM_PLAIN && form != PYSA_AMD_FORM_BLOCKED &&
+      form != PYSA_AMD_FORM_BLOCKED_EXACT && form != PYSA_AMD_FORM_AUTO)
+    return fail(PYSA_AMD_EINVAL, "AIS form must be PLAIN, BLOCKED, BLOCKED_EXACT or AUTO");
+  // The MFMA form: float32, uniform reference, Philox randomness, sequential
+  // or block-random order, 65 <= n <= 2048.
+  const bool mfma_ok = P->dtype == 0 && reference == 0 && !replay_log_r && !replay_perm &&
+                       (order == pysa::kSequential || order == pysa::kBlockRandom) &&
+                       P->n > 64 && P->n <= pysa::kBlockedMaxN;
+  if ((form == PYSA_AMD_FORM_BLOCKED || form == PYSA_AMD_FORM_BLOCKED_EXACT) && !mfma_ok)
+    return fail(PYSA_AMD_ENOTSUP,
+                "MFMA AIS needs float32 couplings, 65 <= n <= 2048, sequential or block-random "
+                "order, the uniform reference and no replay");
+  if (form != PYSA_AMD_FORM_PLAIN && mfma_ok) {
+    std::lock_guard<std::mutex> lk(P->mu);
+    HIP_TRY(hipSetDevice(P->device));
+    hipStream_t st = P->stream;
+    const size_t n = P->n;
+    Staged sg;
+    void *dT, *dS, *dE, *dW;
+    HIP_TRY(sg.up(temps, (size_t)nt * sizeof(float), &dT, st));
+    HIP_TRY(sg.up(states, (size_t)chains * n, &dS, st));
+    HIP_TRY(sg.alloc((size_t)chains * sizeof(double), &dE));
+    HIP_TRY(sg.alloc((size_t)chains * sizeof(double), &dW));
+    // the initial energies in f64 (simulation.py:301)
+    hipError_t le = pysa::launch_energy_f64<float>((const float*)P->dJt, (const float*)P->dh,
+                                                   (const int8_t*)dS, (double*)dE, (int)n, chains, st);
+    if (le != hipSuccess) {
+      (void)hipStreamSynchronize(st);
+      return fail(PYSA_AMD_EHIP, std::string("energy launch: ") + hipGetErrorString(le));
+    }
+    std::string why;
+    double ms = 0.0;
+    const int rc = pysa::run_blocked_ais(P->blocked, P->dJ, P->dJt, P->dh, (int)n, P->ptype, chains,
+                                         nt, (const float*)dT, order, ordering,
+                                         form == PYSA_AMD_FORM_BLOCKED_EXACT ? 3 : 0, seed,
+                                         (int8_t*)dS, (double*)dE, (double*)dW, st, &ms, &why);
+    if (rc != 0) return fail(rc, why);
+    if (kernel_ms) *kernel_ms = ms;
+    HIP_TRY(hipMemcpy(states, dS, (size_t)chains * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(log_w, dW, (size_t)chains * sizeof(double), hipMemcpyDeviceToHost));
+    if (energies)
+      HIP_TRY(hipMemcpy(energies, dE, (size_t)chains * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
   if (P->n > pysa::kPlainMaxN) return fail(PYSA_AMD_ENOTSUP, "n above 4096");
   if (P->dtype == 1 && P->n > pysa::kPlainMaxN64)
     return fail(PYSA_AMD_ENOTSUP, "float64 above n = 2048 is not supported by the plain kernel");

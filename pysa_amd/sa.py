@@ -432,13 +432,18 @@ class Solver(object):
                             seed: int = None,
                             ordering: str = 'reference',
                             replay: dict = None,
-                            engine_stats: dict = None):
+                            engine_stats: dict = None,
+                            form: str = 'auto'):
         """Annealed importance sampling from a uniform or Bernoulli reference
         to the Boltzmann distribution at `beta` (pysa/sa.py:432-553).
 
         Returns (logZf, samples, omegas).  `ordering='reference'` takes each
         weight increment after the sweep at beta_t (pysa/simulation.py:305-316);
-        'neal' takes it before (the unbiased AIS ordering)."""
+        'neal' takes it before (the unbiased AIS ordering).  `form`: 'plain',
+        'blocked', 'blocked_exact' or 'auto'; 'auto' runs the MFMA form for
+        float32 problems of 65 to 2048 variables with the uniform reference
+        and no replay, where update_strategy='random' is the block-random
+        order (as in metropolis_update)."""
         if reference_prob == "uniform":
             logZ0 = _ais.uniform_partition_fun(self.n_vars)
             initialize_prob = _ais.uniform_prob_initialization
@@ -489,6 +494,14 @@ class Solver(object):
                 raise ValueError("replay of update_strategy='random' needs replay['perm'].")
         if seed is None:
             seed = secrets.randbits(64)
+        codes = {'plain': _lib.FORM_PLAIN, 'blocked': _lib.FORM_BLOCKED,
+                 'blocked_exact': _lib.FORM_BLOCKED_EXACT, 'auto': _lib.FORM_AUTO}
+        if form not in codes:
+            raise ValueError(f"form='{form}' not recognized.")
+        mfma = (form != 'plain' and ft == np.float32 and ref_code == 0 and replay is None
+                and 64 < self.n_vars <= 2048)
+        if mfma and order == _lib.ORDER['random']:
+            order = _lib.ORDER['block_random']
 
         prob = _lib.Problem(np.asarray(couplings, dtype=ft), np.asarray(local_fields, dtype=ft),
                             self.problem_type, ft)
@@ -499,7 +512,7 @@ class Solver(object):
                 prob.handle, _lib.ptr(np.ascontiguousarray(temps)), int(num_temps),
                 int(num_samps), ref_code, p, order, 0 if ordering == 'reference' else 1,
                 int(seed) & (2**64 - 1), _lib.ptr(rep_lr), _lib.ptr(rep_perm), _lib.ptr(spins),
-                _lib.ptr(log_omegas), None, _lib.ctypes.byref(kms)))
+                _lib.ptr(log_omegas), None, codes[form], _lib.ctypes.byref(kms)))
         finally:
             prob.close()
         if engine_stats is not None:

@@ -99,7 +99,7 @@ def assert_same(a, b, keys=("states", "energies", "beta_idx", "best_state", "bes
 
 
 def engine_ais(J, h, ptype, temps, states, *, ref=0, p=0.5, order=0, ordering=0, seed=0,
-               replay=None, energies=False):
+               replay=None, energies=False, form=_lib.FORM_PLAIN):
     """AIS through the C ABI; returns (states, log_w) like oracle.ais, and the
     tracked final energies as a third item when `energies`."""
     ft = np.dtype(J.dtype)
@@ -119,7 +119,7 @@ def engine_ais(J, h, ptype, temps, states, *, ref=0, p=0.5, order=0, ordering=0,
         _lib.check(_lib.load().pysa_amd_ais(
             prob.handle, _lib.ptr(temps), len(temps), chains, ref, p, order, ordering,
             seed & (2**64 - 1), _lib.ptr(lr), _lib.ptr(pm), _lib.ptr(st), _lib.ptr(lw),
-            _lib.ptr(en), _lib.ctypes.byref(kms)))
+            _lib.ptr(en), form, _lib.ctypes.byref(kms)))
     finally:
         prob.close()
     return (st, lw, en) if energies else (st, lw)

@@ -147,3 +147,72 @@ def test_ais_float32_matches_oracle_weights():
     same = np.all(got[0] == want[0], axis=1)
     assert same.mean() > 0.9
     assert np.allclose(got[1][same], want[1][same], rtol=1e-6, atol=1e-4)
+
+
+# ---- MFMA form (uniform reference, sequential or block-random order)
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("ordering", [0, 1])
+@pytest.mark.parametrize("order", [0, 3])
+def test_ais_blocked_matches_oracle(ptype, ordering, order):
+    """Integer couplings (exact in every encoding): the MFMA form's states,
+    weights and tracked energies equal the oracle's, including chains that
+    leave a tile partly empty and a size that is not a multiple of 32."""
+    from pysa_amd import _lib
+    rng = np.random.default_rng(900 + 4 * ptype + 2 * ordering + (order == 3))
+    n, chains, nt = 150, 70, 20
+    J, h = int_problem(n, rng)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    temps = np.array([(t + 1) / nt for t in range(nt)], dtype=np.float32)
+    st = random_spins((chains, n), rng, ptype)
+    want = oracle.ais(J, h, ptype, temps, st, order=order, ordering=ordering, seed=321, nthreads=8)
+    got = engine_ais(J, h, ptype, temps, st, order=order, ordering=ordering, seed=321,
+                     energies=True, form=_lib.FORM_BLOCKED)
+    assert np.array_equal(got[0], want[0])
+    assert np.array_equal(got[1], want[1])
+    Jd = J.astype(np.float64)
+    s = got[0].astype(np.float64)
+    e64 = 0.5 * np.sum((s @ Jd) * s, axis=1) + s @ h.astype(np.float64)
+    assert np.array_equal(got[2], e64)
+
+
+def test_ais_blocked_equals_plain_block_random():
+    """Both forms in the block-random order give the same chains (integer
+    couplings, int8 encoding, 2 x 32-spin blocks per tile of a large size)."""
+    from pysa_amd import _lib
+    rng = np.random.default_rng(910)
+    n, chains, nt = 300, 128, 12
+    J, h = int_problem(n, rng, -40, 40, 5)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    temps = np.array([(t + 1) / nt for t in range(nt)], dtype=np.float32)
+    st = random_spins((chains, n), rng, 0)
+    a = engine_ais(J, h, 0, temps, st, order=3, seed=5, form=_lib.FORM_PLAIN)
+    b = engine_ais(J, h, 0, temps, st, order=3, seed=5, form=_lib.FORM_BLOCKED)
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+
+
+def _ring(n, coupling):
+    """Periodic Ising chain: J_{i,i+1} = coupling; log Z in closed form
+    (transfer matrix: Z = (2 cosh J)^n + (2 sinh J)^n at beta = 1, with the
+    reference's E = s.J.s/2 summing each bond once)."""
+    J = np.zeros((n, n), np.float32)
+    for i in range(n):
+        J[i, (i + 1) % n] = J[(i + 1) % n, i] = coupling
+    # E = sum_bonds J s_i s_j, Boltzmann weight exp(-E): Z = (2 cosh J)^n + (2 sinh(-J))^n
+    c = float(np.float32(coupling))
+    logz = n * math.log(2 * math.cosh(c)) + math.log1p((math.tanh(-c)) ** n)
+    return J, np.zeros(n, np.float32), logz
+
+
+@pytest.mark.parametrize("form", ["blocked", "blocked_exact"])
+def test_ais_blocked_log_z_ring(form):
+    """Float couplings (0.37 is not a sum of two bf16 values): log Z of a
+    100-spin periodic chain against the closed form, through the Solver."""
+    J, h, logz = _ring(100, 0.37)
+    P = J + np.diag(h)
+    s = Solver(P, "ising", "float32")
+    stats = {}
+    logzf, _, _ = s.annealed_importance(400, 4096, "uniform", "random", seed=91, form=form,
+                                        engine_stats=stats)
+    assert abs(logzf - logz) < 0.05, (logzf, logz)
