@@ -216,3 +216,16 @@ def test_ais_blocked_log_z_ring(form):
     logzf, _, _ = s.annealed_importance(400, 4096, "uniform", "random", seed=91, form=form,
                                         engine_stats=stats)
     assert abs(logzf - logz) < 0.05, (logzf, logz)
+
+
+def test_ais_blocked_log_z_ring_qubo():
+    """The same ring as a QUBO (x = (s+1)/2: couplings 4J, fields -4J, energy
+    shifted by nJ), which the MFMA form runs with +-1 operands and row sums."""
+    n, c = 100, 0.37
+    J, _, logz_ising = _ring(n, c)
+    Jq = 4 * J
+    hq = np.full(n, -4 * np.float32(c), np.float32)
+    logz = logz_ising + n * float(np.float32(c))
+    s = Solver(Jq + np.diag(hq), "qubo", "float32")
+    logzf, _, _ = s.annealed_importance(400, 4096, "uniform", "random", seed=92, form="blocked")
+    assert abs(logzf - logz) < 0.05, (logzf, logz)
