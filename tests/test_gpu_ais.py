@@ -229,3 +229,24 @@ def test_ais_blocked_log_z_ring_qubo():
     s = Solver(Jq + np.diag(hq), "qubo", "float32")
     logzf, _, _ = s.annealed_importance(400, 4096, "uniform", "random", seed=92, form="blocked")
     assert abs(logzf - logz) < 0.05, (logzf, logz)
+
+
+@pytest.mark.parametrize("chains,temps", [(3, [0.0, 0.25, 1.0]), (65, [0.5]),
+                                          (1, [0.0, 0.0, 0.7, 1.3])])
+def test_ais_blocked_edge_cases(chains, temps):
+    """A tile with few chains, a schedule starting at beta = 0 (every
+    proposal accepted), a single step, repeated betas: the MFMA form equals
+    the oracle bit for bit (integer couplings, block-random order)."""
+    from pysa_amd import _lib
+    rng = np.random.default_rng(920 + chains)
+    n = 97
+    J, h = int_problem(n, rng)
+    J, h = J.astype(np.float32), h.astype(np.float32)
+    temps = np.array(temps, dtype=np.float32)
+    st = random_spins((chains, n), rng, 0)
+    for ordering in (0, 1):
+        want = oracle.ais(J, h, 0, temps, st, order=3, ordering=ordering, seed=17)
+        got = engine_ais(J, h, 0, temps, st, order=3, ordering=ordering, seed=17,
+                         form=_lib.FORM_BLOCKED)
+        assert np.array_equal(got[0], want[0])
+        assert np.array_equal(got[1], want[1])
