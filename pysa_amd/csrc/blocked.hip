@@ -796,7 +796,7 @@ hipError_t launch_blocked(const BlockedProblem& b, const BlockedPlan& pl, const 
                           int32_t* best_sweeps, double* log_omegas, uint64_t* accepted,
                           void* scratch_mem, hipStream_t st) {
   const size_t cnt = (size_t)reads * R;
-  BParams p;
+  BParams p{};
   p.n = n;
   p.nblk = pl.nblk;
   p.ptype = ptype;
