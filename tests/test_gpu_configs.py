@@ -113,7 +113,7 @@ def test_c3_energy_error_full_length():
     instance with update_strategy='random': tracked and best energies
     within 1e-5 of f64 for both forms (the plain form resets its energy
     with each f64 field refresh, the MFMA form in a dry recomputation pass
-    every 64 sweeps)."""
+    every 256 sweeps)."""
     P, egs = instances.planted_wishart(1024, 0.5, 1003)
     J, h = split(P)
     reads, R, S = 4, 32, 5000

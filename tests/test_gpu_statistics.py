@@ -114,5 +114,7 @@ def test_qubo_forms_same_distribution_ks():
             out[form][k] = 0.5 * np.einsum("ki,ij,kj->k", x, Jd, x) + x @ hd
     for k in (0, 16):
         a, b = out[_lib.FORM_PLAIN][k], out[_lib.FORM_BLOCKED][k]
-        pv = stats.ks_2samp(a, b).pvalue
+        # asymptotic p-value (the exact one is not computable with the ties
+        # of a nearly discrete distribution)
+        pv = stats.ks_2samp(a, b, method="asymp").pvalue
         assert pv > 1e-3, (k, pv, a.mean(), b.mean())
