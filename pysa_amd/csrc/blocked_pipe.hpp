@@ -679,8 +679,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           const int xg = (int)(xm >> 2);
           // visit group r4 holds block rows 4 (r4 ^ xg) + (e ^ (xm & 3)); the
           // in-group part is a compile-time permutation per uniform branch
-          auto load_fields = [&](auto pc) __attribute__((always_inline)) {
+          auto load_fields = [&](auto pc, auto rsc) __attribute__((always_inline)) {
             constexpr int P = decltype(pc)::value;
+            constexpr bool RS = decltype(rsc)::value;  // row sums in the fields (QUBO, bf16)
             // kFG groups of 4 rows per stage: every LDS read of a stage is
             // issued before its first sum, so a stage costs one LDS latency
 #pragma unroll
@@ -695,7 +696,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                   ld[j][mm] = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
                 ld[j][MW] = *reinterpret_cast<const float4*>(&L.corr[o]);
                 ld[j][MW + 1] = *reinterpret_cast<const float4*>(&L.hb[pb][tq][4 * gr]);
-                ld[j][MW + 2] = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
+                if constexpr (RS) ld[j][MW + 2] = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
               }
               __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -708,9 +709,12 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                   sm.z = sm.z + ld[j][mm].z;
                   sm.w = sm.w + ld[j][mm].w;
                 }
-                const float4 cr = ld[j][MW], h4 = ld[j][MW + 1], s4 = ld[j][MW + 2];
-                const float vs[4] = {sm.x + cr.x + s4.x, sm.y + cr.y + s4.y,
-                                     sm.z + cr.z + s4.z, sm.w + cr.w + s4.w};
+                const float4 cr = ld[j][MW], h4 = ld[j][MW + 1];
+                float vs[4] = {sm.x + cr.x, sm.y + cr.y, sm.z + cr.z, sm.w + cr.w};
+                if constexpr (RS) {
+                  const float4 s4 = ld[j][MW + 2];
+                  vs[0] = vs[0] + s4.x, vs[1] = vs[1] + s4.y, vs[2] = vs[2] + s4.z, vs[3] = vs[3] + s4.w;
+                }
                 const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -725,12 +729,24 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
               __builtin_amdgcn_sched_barrier(0);
             }
           };
-          switch (xm & 3u) {
-            case 0: load_fields(std::integral_constant<int, 0>{}); break;
-            case 1: load_fields(std::integral_constant<int, 1>{}); break;
-            case 2: load_fields(std::integral_constant<int, 2>{}); break;
-            default: load_fields(std::integral_constant<int, 3>{}); break;
+          using RS1 = std::integral_constant<bool, true>;
+          using RS0 = std::integral_constant<bool, false>;
+          if (qsum) {
+            switch (xm & 3u) {
+              case 0: load_fields(std::integral_constant<int, 0>{}, RS1{}); break;
+              case 1: load_fields(std::integral_constant<int, 1>{}, RS1{}); break;
+              case 2: load_fields(std::integral_constant<int, 2>{}, RS1{}); break;
+              default: load_fields(std::integral_constant<int, 3>{}, RS1{}); break;
+            }
+          } else {
+            switch (xm & 3u) {
+              case 0: load_fields(std::integral_constant<int, 0>{}, RS0{}); break;
+              case 1: load_fields(std::integral_constant<int, 1>{}, RS0{}); break;
+              case 2: load_fields(std::integral_constant<int, 2>{}, RS0{}); break;
+              default: load_fields(std::integral_constant<int, 3>{}, RS0{}); break;
+            }
           }
+
         }
         // spins in visit order (bit k = block row k ^ xm)
         uint32_t mb = xor_permute_bits(L.bb[pb][qo], xm);
