@@ -822,6 +822,7 @@ hipError_t launch_blocked(const BlockedProblem& b, const BlockedPlan& pl, const 
   // problem (0 = never), a switch for tests and comparisons.
   const char* ev = getenv("PYSA_AMD_ENERGY_REFRESH");
   p.refresh = ev && ev[0] ? atoi(ev) : (b.exact_sums == 1 ? 0 : 256);
+  p.hzero = b.h_zero == 1 ? 1 : 0;
   p.rowsum = (!pl.i8 && ptype == 1) ? b.rowsum : nullptr;
   p.betas = (const float*)betas;
   p.bsort = bsort;
@@ -958,6 +959,9 @@ int run_blocked(BlockedProblem& b, const void* dJ, const void* dJt, const void* 
     }
     b.exact_kind = blocked_exact_pieces(hJ.data(), hJ.size());
     b.exact_sums = integer_sums_exact(hJ.data(), hh.data(), n) ? 1 : 0;
+    b.h_zero = 1;
+    for (int i = 0; i < n; ++i)
+      if (hh[i] != 0.f) b.h_zero = 0;
   }
   const BlockedPlan pl = plan_blocked(b, n, reads, R, order, pieces);
   int rc = prepare_encoding(b, dJ, dJt, n, ptype, pl, st, why);
@@ -1018,6 +1022,9 @@ int run_blocked_ais(BlockedProblem& b, const void* dJ, const void* dJt, const vo
     }
     b.exact_kind = blocked_exact_pieces(hJ.data(), hJ.size());
     b.exact_sums = integer_sums_exact(hJ.data(), hh.data(), n) ? 1 : 0;
+    b.h_zero = 1;
+    for (int i = 0; i < n; ++i)
+      if (hh[i] != 0.f) b.h_zero = 0;
   }
   // one chain per read (R = 1: the chain index is the oracle's), two tiles
   // of 32 chains per workgroup on the pipelined kernel
@@ -1043,6 +1050,7 @@ int run_blocked_ais(BlockedProblem& b, const void* dJ, const void* dJt, const vo
   // dry passes recompute the energy every 64 steps unless every sum is exact
   const char* ev = getenv("PYSA_AMD_ENERGY_REFRESH");
   p.refresh = ev && ev[0] ? atoi(ev) : (b.exact_sums == 1 ? 0 : 64);
+  p.hzero = b.h_zero == 1 ? 1 : 0;
   p.kb = pl.kbp;
   p.k0 = (uint32_t)seed;
   p.k1 = (uint32_t)(seed >> 32);

@@ -22,6 +22,7 @@ struct BlockedProblem {
   int packed_kind = -1;  // 0 int8, 1..3 bf16 pieces
   int exact_kind = -1;   // smallest exact encoding of this J
   int exact_sums = -1;   // 1: J and h integers with every field sum exact in f32
+  int h_zero = -1;       // 1: every local field is zero
   int last_kind = -1;    // encoding used by the last run
   void* scratch = nullptr;
   size_t scratch_bytes = 0;

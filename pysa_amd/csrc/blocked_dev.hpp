@@ -131,6 +131,7 @@ struct BParams {
   int order;  // kSequential or kBlockRandom (pipelined kernel only)
   int refresh;  // pipelined kernel: sweeps between f64 energy recomputations (0 = never)
   int kb;  // row blocks of the K range each wave of a group owns
+  int hzero;  // every local field is zero (the field sums skip h)
   uint32_t k0, k1;
   uint64_t sweep_offset;
   const void* frag;

@@ -679,9 +679,13 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           const int xg = (int)(xm >> 2);
           // visit group r4 holds block rows 4 (r4 ^ xg) + (e ^ (xm & 3)); the
           // in-group part is a compile-time permutation per uniform branch
-          auto load_fields = [&](auto pc, auto rsc) __attribute__((always_inline)) {
+          // FM: 0 = sums + h; 1 = (sums + row sums) / 2 + h (QUBO with bf16
+          // operands); 2 = the sums alone (every h zero: adding +0 would only
+          // turn a -0 into +0, which no decision or sum distinguishes)
+          auto load_fields = [&](auto pc, auto fmc) __attribute__((always_inline)) {
             constexpr int P = decltype(pc)::value;
-            constexpr bool RS = decltype(rsc)::value;  // row sums in the fields (QUBO, bf16)
+            constexpr int FM = decltype(fmc)::value;
+            constexpr bool RS = FM == 1;
             // kFG groups of 4 rows per stage: every LDS read of a stage is
             // issued before its first sum, so a stage costs one LDS latency
 #pragma unroll
@@ -695,7 +699,7 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                 for (int mm = 0; mm < MW; ++mm)
                   ld[j][mm] = *reinterpret_cast<const float4*>(&L.part[pb][mm][o]);
                 ld[j][MW] = *reinterpret_cast<const float4*>(&L.corr[o]);
-                ld[j][MW + 1] = *reinterpret_cast<const float4*>(&L.hb[pb][tq][4 * gr]);
+                if constexpr (FM != 2) ld[j][MW + 1] = *reinterpret_cast<const float4*>(&L.hb[pb][tq][4 * gr]);
                 if constexpr (RS) ld[j][MW + 2] = *reinterpret_cast<const float4*>(&L.rsb[pb][4 * gr]);
               }
               __builtin_amdgcn_sched_barrier(0);
@@ -709,7 +713,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                   sm.z = sm.z + ld[j][mm].z;
                   sm.w = sm.w + ld[j][mm].w;
                 }
-                const float4 cr = ld[j][MW], h4 = ld[j][MW + 1];
+                const float4 cr = ld[j][MW];
+                const float4 h4 = FM != 2 ? ld[j][MW + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
                 float vs[4] = {sm.x + cr.x, sm.y + cr.y, sm.z + cr.z, sm.w + cr.w};
                 if constexpr (RS) {
                   const float4 s4 = ld[j][MW + 2];
@@ -722,30 +727,30 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
                     fp[4 * (g0 + j) + e] = vs[e ^ P];
                     hk[4 * (g0 + j) + e] = hv[e ^ P];
                   } else {
-                    fp[4 * (g0 + j) + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
+                    if constexpr (FM == 2)
+                      fp[4 * (g0 + j) + e] = vs[e ^ P];
+                    else
+                      fp[4 * (g0 + j) + e] = __builtin_fmaf(fsc, vs[e ^ P], hv[e ^ P]);
                   }
                 }
               }
               __builtin_amdgcn_sched_barrier(0);
             }
           };
-          using RS1 = std::integral_constant<bool, true>;
-          using RS0 = std::integral_constant<bool, false>;
-          if (qsum) {
+          auto load_fm = [&](auto fmc) __attribute__((always_inline)) {
             switch (xm & 3u) {
-              case 0: load_fields(std::integral_constant<int, 0>{}, RS1{}); break;
-              case 1: load_fields(std::integral_constant<int, 1>{}, RS1{}); break;
-              case 2: load_fields(std::integral_constant<int, 2>{}, RS1{}); break;
-              default: load_fields(std::integral_constant<int, 3>{}, RS1{}); break;
+              case 0: load_fields(std::integral_constant<int, 0>{}, fmc); break;
+              case 1: load_fields(std::integral_constant<int, 1>{}, fmc); break;
+              case 2: load_fields(std::integral_constant<int, 2>{}, fmc); break;
+              default: load_fields(std::integral_constant<int, 3>{}, fmc); break;
             }
-          } else {
-            switch (xm & 3u) {
-              case 0: load_fields(std::integral_constant<int, 0>{}, RS0{}); break;
-              case 1: load_fields(std::integral_constant<int, 1>{}, RS0{}); break;
-              case 2: load_fields(std::integral_constant<int, 2>{}, RS0{}); break;
-              default: load_fields(std::integral_constant<int, 3>{}, RS0{}); break;
-            }
-          }
+          };
+          if (qsum)
+            load_fm(std::integral_constant<int, 1>{});
+          else if (!BAT && p.hzero)
+            load_fm(std::integral_constant<int, 2>{});
+          else
+            load_fm(std::integral_constant<int, 0>{});
 
         }
         // spins in visit order (bit k = block row k ^ xm)

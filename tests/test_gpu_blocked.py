@@ -221,3 +221,27 @@ def test_sweep_counter_past_32_bits(form, order):
     low = engine_metropolis(J, h, 0, betas, st, en, S, order=order, seed=5,
                             sweep_offset=off - (1 << 32) + (1 << 33), form=form)
     assert not np.array_equal(got["states"], low["states"])
+
+
+@pytest.mark.parametrize("ptype", [0, 1])
+@pytest.mark.parametrize("order", [0, 3])
+@pytest.mark.parametrize("kind", ["int", "bf16"])
+def test_zero_fields_bit_exact(ptype, order, kind):
+    """All local fields zero (the field sums then skip h): the MFMA form
+    equals the oracle bit for bit, with int8 couplings and with couplings
+    that take two bf16 pieces."""
+    rng = np.random.default_rng(2400 + 4 * ptype + 2 * (order == 3) + (kind == "bf16"))
+    n, R, reads, S = 300, 16, 6, 25
+    # integers up to 3000 need two bf16 pieces; up to 9, int8
+    big = kind == "bf16"
+    J, _ = int_problem(n, rng, -3000, 3000, 0) if big else int_problem(n, rng, -9, 9, 0)
+    J = J.astype(np.float32)
+    h = np.zeros(n, np.float32)
+    scale = 330.0 if big else 1.0
+    betas = (1 / (scale * np.geomspace(8.0, 60.0, R))).astype(np.float32)
+    st = random_spins((reads, R, n), rng, ptype)
+    en = energies_of(J, h, st, ptype, np.float32)
+    ref = oracle.metropolis(J, h, ptype, betas, st, en, S, order=order, seed=81, nthreads=8)
+    got = engine_metropolis(J, h, ptype, betas, st, en, S, order=order, seed=81,
+                            form=_lib.FORM_BLOCKED)
+    assert_same(got, ref)
