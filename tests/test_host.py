@@ -160,3 +160,18 @@ def test_rbm_log_z_matches_enumeration():
     E = 0.5 * np.einsum("ki,ij,kj->k", s, J, s) + s @ h
     for beta in (1.0, 0.37):
         assert np.isclose(instances.rbm_log_z(P, nv, beta), logsumexp(-beta * E), rtol=1e-12)
+
+
+def test_batched_and_ais_form_argument_errors():
+    """Argument checks that run before any device call."""
+    from pysa_amd import batched
+    rng = np.random.default_rng(3)
+    A = np.triu(2 * rng.integers(0, 2, (40, 40)) - 1, 1)
+    inst = [((A + A.T).astype(np.int8), np.float32(0.2))]
+    with pytest.raises(ValueError, match="form='bogus' not recognized"):
+        batched.solve_batch(inst, num_sweeps=2, form="bogus")
+    with pytest.raises(ValueError, match="update_strategy='shuffled' not recognized"):
+        batched.solve_batch(inst, num_sweeps=2, update_strategy="shuffled")
+    s = Solver(sym(6, rng), "ising")
+    with pytest.raises(ValueError, match="form='bogus' not recognized"):
+        s.annealed_importance(5, 4, "uniform", form="bogus")
