@@ -292,7 +292,8 @@ def sparse_lattice(L=32, reads=512, R=32, sweeps=100, seed=1006):
     (T 0.5 -> 3.0): same results bit for bit, device time of one pass each.
     The sparse form reads 4 (column, value) pairs, 32 B, per accepted flip
     against n x 4 B for the dense row, but runs half the waves (its update
-    needs more registers), and the per-visit work, not the row, bounds both."""
+    needs more registers), and the per-visit work, not the row, bounds both.
+    The MFMA form (int8 products of the dense +-1 matrix) is timed as well."""
     from pysa_amd import _lib
 
     rng = np.random.default_rng(seed)
@@ -315,6 +316,12 @@ def sparse_lattice(L=32, reads=512, R=32, sweeps=100, seed=1006):
         log(f"[bench] lattice {name}: {ms:.1f} ms device")
     res["sparse_over_plain"] = res["sparse"]["value"] / res["plain"]["value"]
     res["same_states"] = bool(np.array_equal(states["plain"], states["sparse"]))
+    # the MFMA form on the same lattice (int8 couplings, block-random order):
+    # a +-1 lattice is an exact int8 matrix, so the dense products win
+    wl.step(_lib.FORM_BLOCKED, _lib.ORDER["block_random"], seed)
+    ms = wl.step(_lib.FORM_BLOCKED, _lib.ORDER["block_random"], seed + 1)
+    res["blocked"] = {"value": wl.updates / (ms * 1e-3), "ms": ms, "order": "block_random"}
+    log(f"[bench] lattice blocked: {ms:.1f} ms device")
     return res
 
 
