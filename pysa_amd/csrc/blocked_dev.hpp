@@ -24,14 +24,17 @@ typedef __attribute__((ext_vector_type(4))) unsigned u32x4v;
 constexpr int kB = 32;  // block size (rows of one MFMA tile)
 
 // x with bit k moved to bit k ^ m (m < 32, wave-uniform): the in-block
-// permutation of the block-random order, an involution.
+// permutation of the block-random order, an involution.  Bits 0-2 of m: one
+// exchange of bit pairs d apart per bit, shifted by m & d (a shift by 0
+// leaves x as it is, so no select); bits 3-4: the bytes exchanged by one
+// byte permute (byte i from byte i ^ (m >> 3)).
 __device__ __forceinline__ uint32_t xor_permute_bits(uint32_t x, uint32_t m) {
-  if (m & 1u) x = ((x & 0x55555555u) << 1) | ((x >> 1) & 0x55555555u);
-  if (m & 2u) x = ((x & 0x33333333u) << 2) | ((x >> 2) & 0x33333333u);
-  if (m & 4u) x = ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
-  if (m & 8u) x = ((x & 0x00FF00FFu) << 8) | ((x >> 8) & 0x00FF00FFu);
-  if (m & 16u) x = (x << 16) | (x >> 16);
-  return x;
+  const uint32_t s1 = m & 1u, s2 = m & 2u, s4 = m & 4u;
+  x = ((x << s1) & 0xAAAAAAAAu) | ((x >> s1) & 0x55555555u);
+  x = ((x << s2) & 0xCCCCCCCCu) | ((x >> s2) & 0x33333333u);
+  x = ((x << s4) & 0xF0F0F0F0u) | ((x >> s4) & 0x0F0F0F0Fu);
+  const uint32_t sel = 0x03020100u ^ (((m >> 3) & 3u) * 0x01010101u);
+  return __builtin_amdgcn_perm(x, x, sel);
 }
 
 // ------------------------------------------------------------ device util --

@@ -59,12 +59,11 @@ struct __align__(16) PipeLds {
   // kHL floats so that 16-byte reads of consecutive replicas hit distinct banks
   float lo[2][64][kHL];
   float hi[2][64][kHL];
-  uint32_t bb[2][64];         // block spins before the block (bit r = row r)
+  uint32_t bb[2][64];         // block spins before the block (bit k = row k ^ the visit mask)
   uint32_t bnew[2][64];       // block spins after the block (by block parity)
   float sfp[64 * 33];         // exact rerun staging
-  // visit schedule by pass parity: block at step t and its in-block XOR mask
-  uint8_t ord[2][64];
-  uint8_t msk[2][64];
+  // visit schedule by pass parity: block at step t | its in-block XOR mask << 8
+  uint16_t sch[2][64];
   float E[64];
   float rb[64];
   float bet[64];  // the ladder by slot (batched: tile t's instance's at 32 t)
@@ -449,13 +448,16 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
     }
   };
   auto owner_of = [&](int c) { return 1 + c / kb; };  // matrix wave owning K block c
-  auto publish_bits = [&](int c, int pb) {  // by the owner of block c
+  // by the owner of block c, in the visit order of its next visit (mask xm),
+  // so that the proposal chain starts without a permute of its own
+  auto publish_bits = [&](int c, int pb, uint32_t xm) {
     const int j = c - c0;
 #pragma unroll
     for (int tt = 0; tt < NT; ++tt) {
       const uint32_t mine = get_half<NW>(sa[tt], j);
       const uint32_t got = (uint32_t)__shfl_xor((int)mine, 32, 64);  // the other half
-      if (lane < 32) L.bb[pb][qtt[tt]] = block_of<I8>(mine, got);
+      const uint32_t bits = block_of<I8>(mine, got);
+      if (lane < 32) L.bb[pb][qtt[tt]] = BR ? xor_permute_bits(bits, xm) : bits;
     }
   };
 
@@ -470,20 +472,18 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         bo = (int)feistel_perm(fk, (uint32_t)lane);
         mk = block_mask(swn, (uint32_t)bo, p.k0, p.k1);
       }
-      L.ord[par][lane] = (uint8_t)bo;
-      L.msk[par][lane] = (uint8_t)mk;
+      L.sch[par][lane] = (uint16_t)(bo | (mk << 8));
     }
   };
+  // The entry of a step: block (low byte) and mask, one LDS read for both.
   // Reads are volatile so that each use reloads the entry (a short LDS read)
   // instead of keeping it live in scalar registers across the block's work.
-  auto sched_block = [&](int par, int step) {
-    if (!BR) return step;
-    return __builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.ord[par][step]);
+  auto sched_entry = [&](int par, int step) -> uint32_t {
+    if (!BR) return (uint32_t)step;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)*(volatile uint16_t*)&L.sch[par][step]);
   };
-  auto sched_mask = [&](int par, int step) {
-    if (!BR) return 0u;
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)*(volatile uint8_t*)&L.msk[par][step]);
-  };
+  auto entry_block = [](uint32_t e) { return BR ? (int)(e & 0xffu) : (int)e; };
+  auto entry_mask = [](uint32_t e) { return BR ? e >> 8 : 0u; };
 
   // Passes over the blocks.  Every `refresh` sweeps (p.refresh > 0 and
   // S > refresh) a dry pass recomputes the energy: the tracked energy adds
@@ -532,13 +532,14 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
     int s0;
     (void)pass_info(0, s0);
     const uint64_t sw0 = (uint64_t)s0 + p.sweep_offset;
-    const int cf = sched_block(0, 0), cs = sched_block(0, 1);
-    const uint32_t mf = sched_mask(0, 0), ms = sched_mask(0, 1);
+    const uint32_t ef = sched_entry(0, 0), es = sched_entry(0, 1);
+    const int cf = entry_block(ef), cs = entry_block(es);
+    const uint32_t mf = entry_mask(ef), ms = entry_mask(es);
     if (mw) {
       prefetch_frags(cf);
       compute_part(cf, -1, L.part[0][m]);
       prefetch_frags(cs);
-      if (w == owner_of(cf)) publish_bits(cf, 0);
+      if (w == owner_of(cf)) publish_bits(cf, 0, mf);
     }
     if (hw) {
       prefetch_jb(cf, mf);
@@ -598,8 +599,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   // sweep).
   if (w == 0) {
     // schedule entries of the current step, each read one block ahead
-    int c_nx = sched_block(0, 0);
-    uint32_t xm_nx = sched_mask(0, 0);
+    const uint32_t e0 = sched_entry(0, 0);
+    int c_nx = entry_block(e0);
+    uint32_t xm_nx = entry_mask(e0);
     for (int ps = 0; ps < npass; ++ps) {
       int s;
       const bool dry = pass_info(ps, s);
@@ -620,13 +622,15 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         const uint32_t xm = xm_nx;
         const int pos0 = 32 * c;
         const bool lastb = tb + 1 == nblk;
-        const int c1 = sched_block(lastb ? sp ^ 1 : sp, lastb ? 0 : tb + 1);
-        xm_nx = sched_mask(lastb ? sp ^ 1 : sp, lastb ? 0 : tb + 1);
+        const uint32_t e1 = sched_entry(lastb ? sp ^ 1 : sp, lastb ? 0 : tb + 1);
+        const int c1 = entry_block(e1);
+        xm_nx = entry_mask(e1);
         c_nx = c1;
         FragSet fcorr;
         load_corr_frags(c1, c, fcorr);
         if (dry) {
           // ---- dry pass: f64 energy of the block's rows from its full fields
+          // (a dry pass visits every block with mask 0: bits in row order)
           const uint32_t bits = L.bb[pb][qo];
           for (int i = 0; i < kB; ++i) {
             if (pos0 + i < nq) {
@@ -641,7 +645,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             }
           }
           L.bnew[pb][qo] = bits;
-          if (c1 == c) L.bb[pb ^ 1][qo] = bits;  // the same block next (the owners skip it)
+          // the same block next (the owners skip it), in its next visit order
+          if (c1 == c) L.bb[pb ^ 1][qo] = BR ? xor_permute_bits(bits, xm_nx) : bits;
           corr_products(fcorr, bits, qo);
           __syncthreads();  // B
           if (lastb) {
@@ -753,8 +758,8 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
             load_fm(std::integral_constant<int, 0>{});
 
         }
-        // spins in visit order (bit k = block row k ^ xm)
-        uint32_t mb = xor_permute_bits(L.bb[pb][qo], xm);
+        // spins in visit order (bit k = block row k ^ xm), as published
+        uint32_t mb = L.bb[pb][qo];
         const uint32_t mb0 = mb;
         // accepted flips, step k at bit 31 - k: one add with carry per step
         uint32_t tk = 0;
@@ -864,11 +869,11 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
           }
         }
         {
-          const uint32_t mbr = xor_permute_bits(mb, xm);  // back to block-row order
+          const uint32_t mbr = BR ? xor_permute_bits(mb, xm) : mb;  // back to block-row order
           L.bnew[pb][qo] = mbr;
-          // the same block next (at a pass boundary): its spins are these
-          // (the owners, whose copy is older, skip it)
-          if (c1 == c) L.bb[pb ^ 1][qo] = mbr;
+          // the same block next (at a pass boundary): its spins are these,
+          // in its next visit order (the owners, whose copy is older, skip it)
+          if (c1 == c) L.bb[pb ^ 1][qo] = BR ? xor_permute_bits(mbr, xm_nx) : mbr;
           corr_products(fcorr, mbr, qo);
         }
         if constexpr (!BAT) dsum += (double)dpart;
@@ -994,8 +999,9 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
   } else {
     // schedule entries carried from block to block: c (this step), c1 (the
     // next) and its mask; c2 (the one after) is read each block
-    int c = sched_block(0, 0), c1 = sched_block(0, 1);
-    uint32_t m1 = sched_mask(0, 1);
+    const uint32_t e1 = sched_entry(0, 1);
+    int c = entry_block(sched_entry(0, 0)), c1 = entry_block(e1);
+    uint32_t m1 = entry_mask(e1);
     for (int ps = 0; ps < npass; ++ps) {
       int s, s_next;
       const bool dry = pass_info(ps, s);
@@ -1009,13 +1015,14 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         if (BR && w == 4 && tb == 0) fill_sched(sp ^ 1, sw_next, dry_next);
         const bool last = tb + 1 == nblk;
         const int p2 = tb + 2 < nblk ? sp : sp ^ 1, i2 = tb + 2 < nblk ? tb + 2 : tb + 2 - nblk;
-        const int c2 = sched_block(p2, i2);
-        const uint32_t m2 = sched_mask(p2, i2);
+        const uint32_t en2 = sched_entry(p2, i2);
+        const int c2 = entry_block(en2);
+        const uint32_t m2 = entry_mask(en2);
         // ---- meanwhile: block c1's spins, fields except K block c, J / h and brackets
         if (mw) {
           // c1's spins have not changed since its last visit unless c1 == c
           // (a pass boundary), which the proposal wave publishes itself
-          if (w == owner_of(c1) && c1 != c) publish_bits(c1, pn);
+          if (w == owner_of(c1) && c1 != c) publish_bits(c1, pn, m1);
           compute_part(c1, c, L.part[pn][m]);
           prefetch_frags(c2);
         }
