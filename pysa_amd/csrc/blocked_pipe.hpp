@@ -1015,20 +1015,32 @@ __global__ __launch_bounds__(128 * (MW + 1)) void blocked_pipe_kernel(BParams p0
         if (BR && w == 4 && tb == 0) fill_sched(sp ^ 1, sw_next, dry_next);
         const bool last = tb + 1 == nblk;
         const int p2 = tb + 2 < nblk ? sp : sp ^ 1, i2 = tb + 2 < nblk ? tb + 2 : tb + 2 - nblk;
-        const uint32_t en2 = sched_entry(p2, i2);
-        const int c2 = entry_block(en2);
-        const uint32_t m2 = entry_mask(en2);
+        // The entry two steps ahead is read here and made wave-uniform where
+        // it is first needed (the prefetches at the end of the step), so the
+        // step's work does not wait for the LDS read.
+        const uint32_t raw2 = BR ? (uint32_t)*(volatile uint16_t*)&L.sch[p2][i2] : (uint32_t)i2;
+        auto entry2 = [&]() {
+          return BR ? (uint32_t)__builtin_amdgcn_readfirstlane((int)raw2) : raw2;
+        };
+        int c2 = 0;
+        uint32_t m2 = 0;
         // ---- meanwhile: block c1's spins, fields except K block c, J / h and brackets
         if (mw) {
           // c1's spins have not changed since its last visit unless c1 == c
           // (a pass boundary), which the proposal wave publishes itself
           if (w == owner_of(c1) && c1 != c) publish_bits(c1, pn, m1);
           compute_part(c1, c, L.part[pn][m]);
+          const uint32_t en2 = entry2();
+          c2 = entry_block(en2);
+          m2 = entry_mask(en2);
           prefetch_frags(c2);
         }
         if (hw) {
           publish_block(pn);
           if (w > MW + 1 && !last && !dry) brackets(c1, m1, sw, pn);
+          const uint32_t en2 = entry2();
+          c2 = entry_block(en2);
+          m2 = entry_mask(en2);
           prefetch_jb(c2, m2);
           // the sweep's exchange uniforms (utils.py:57), by a helper that has
           // no brackets to compute during the last block
