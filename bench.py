@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # this script (FETCH_SIZE scaled by 2.000 and WRITE_SIZE by 1.000, the factors
 # a 256 MiB device-to-device copy gives); summary in
 # profiles/wishart_n1024_kernels.md.
-MEASURED_TRAFFIC = {("blocked", "block_random", 512, 32, 5000): 195.5e9}
+MEASURED_TRAFFIC = {("blocked", "block_random", 512, 32, 5000): 184.0e9}
 BF16_PEAK_TFLOPS = 2500.0
 FP8_PEAK_TFLOPS = 5000.0
 
